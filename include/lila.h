@@ -1,0 +1,162 @@
+/*
+ * lila.h -- drop-in C ABI of the LILA Householder-QR hot path, implemented
+ * on one AMD MI355X (gfx950) by liblila_hip.so.
+ *
+ * The prototypes below keep the argument lists of the reference header
+ * (lila-c/lila.h, plus lila_dorghr / lila_dgetrf_b from c3/lila.h:13,43), so
+ * a caller of the reference links against liblila_hip.so unchanged.  Each
+ * entry cites the reference declaration it replaces.
+ *
+ * Conventions (as in the reference): column-major FP64; global index i means
+ * the routine works on A(i:m-1, i:i+n-1); T is mt x (i+n) with ldt = mt and
+ * mt-aligned upper-triangular diagonal blocks (block b holds reflectors
+ * [b*mt, (b+1)*mt)); Q is m x (i+n); work / lwork keep their meaning but the
+ * library uses its own device workspace, so any work pointer (or NULL) is
+ * accepted.
+ *
+ * Pointers may be host memory (the library copies the touched span in and
+ * out and returns after the copy-out) or device memory (hipMalloc / a torch
+ * tensor: the call is asynchronous on the library stream, like a BLAS call).
+ *
+ * Return value: 0 on success, a positive LAPACK-style info on numerical
+ * breakdown (Cholesky not positive definite: the 1-based column), negative
+ * on a device or argument error.  The reference always returns 0.
+ * Not thread-safe: one calling thread per process.
+ */
+#ifndef LILA_H_MI355X
+#define LILA_H_MI355X
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* lila-c/lila.h:9 -- recursive panel QR (Elmroth-Gustavson); T n x n upper */
+int dgeqr3(int m, int n, double *A, int lda, double *T, int ldt);
+
+/* lila-c/lila.h:30 -- A(:, j:j+n) <- Q^T A(:, j:j+n), k reflectors from i */
+int lila_dormqrf_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                     int ldt, double *work, int lwork);
+/* lila-c/lila.h:31 -- B(:, j:j+n) <- Q^T B(:, j:j+n) */
+int lila_dormqrf_z03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                     int ldt, double *B, int ldb, double *work, int lwork);
+/* lila-c/lila.h:34 -- Q(:, j:j+n) <- H_i..H_{i+k-1} [0; Q(:, j:j+n)].
+ * Positional order (A, Q, T) is that of the definition
+ * (lila-c/lila_dormqrbz_w03.c:3); the reference header swaps the names. */
+int lila_dormqrbz_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *Q,
+                      int ldq, double *T, int ldt, double *work, int lwork);
+
+/* lila-c/lila.h:38 -- sign-chosen LU reconstruction of block [j, j+n) */
+int lila_dorgh2(int m, int n, int i, int j, int mt, double *A, int lda, double *T, int ldt,
+                double *Q, int ldq, double *work, int lwork, double *S);
+/* lila-c/lila.h:39 */
+int lila_dorgh2_3(int m, int n, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                  double *work, int lwork, double *S);
+/* lila-c/lila.h:40 */
+int lila_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int lda, double *T,
+                       int ldt, double *Q, int ldq, double *work, int lwork, double *S);
+/* c3/lila.h:43 -- blocked Householder reconstruction of A(i:m, i:i+n) */
+int lila_dorghr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                int ldq, double *work, int lwork, int *S);
+/* c3/lila.h:13 -- sign-chosen unpivoted LU; work holds R (n x n, ld n) */
+int lila_dgetrf_b(int m, int n, int i, int mt, double *A, int lda, double *Q, int ldq, double *work,
+                  int lwork);
+
+/* lila-c/lila.h:43 */
+int lila_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int lda, double *T,
+                            int ldt);
+/* lila-c/lila.h:44 */
+int lila_dlarft_w03(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                    double *tau);
+
+/* lila-c/lila.h:46-56 */
+int lila_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n, int i,
+                           int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                           double *work, int lwork);
+int lila_dgeqrf_w03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqr2_w03(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                    int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqrf_w03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                       int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqrf_w03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                          double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqrf_w03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                         double *Q, int ldq, double *work, int lwork);
+int lila_dgeqrf_w03_mt_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                          double *Q, int ldq, double *work, int lwork);
+int lila_dgeqr2_w03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                      double *Q, int ldq, double *work, int lwork);
+int lila_dgeqr2_w03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                      double *Q, int ldq, double *work, int lwork);
+int lila_dgeqr2_w03_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                       double *Q, int ldq, double *work, int lwork);
+
+/* lila-c/lila.h:58-72 -- workspace queries (doubles), the reference's
+ * closed forms; lwork carries the running maximum as in the reference */
+int lila_query_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n,
+                                 int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                                 int ldq, double *work, int lwork);
+int lila_query_dgeqrf_w03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                    double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_query_dgeqrf_ker_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                    double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_query_dgeqrf_w03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                             double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_query_dgeqrf_w03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_query_dormqrbz_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *Q,
+                            int ldq, double *T, int ldt, double *work, int lwork);
+int lila_query_dgeqrf_w03_mt_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                                double *Q, int ldq, double *work, int lwork);
+int lila_query_dgeqrf_w03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                               double *Q, int ldq, double *work, int lwork);
+int lila_query_dgeqr2_w03_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                             double *Q, int ldq, double *work, int lwork);
+int lila_query_dgeqr2_w03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                            double *Q, int ldq, double *work, int lwork);
+int lila_query_dgeqr2_w03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                            double *Q, int ldq, double *work, int lwork);
+int lila_query_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int lda, double *T,
+                             int ldt, double *Q, int ldq, double *work, int lwork, double *S);
+int lila_query_dorgh2_3(int m, int n, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                        double *work, int lwork, double *S);
+int lila_query_dormqrf_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                           int ldt, double *work, int lwork);
+int lila_query_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A, int lda, double *Q,
+                                        int ldq, double *T, int ldt, double *work, int lwork);
+
+/* lila-c/lila.h:92-93 */
+int lila_dT2tau_w03(int m, int n, int i, int mt, double *T, int ldt, double *tau);
+int lila_dV2tau_w03(int m, int n, int i, int mt, double *A, int lda, double *tau);
+
+/* ---- extensions (no counterpart in the reference header) ------------- */
+
+/* Status of the last call (0 ok, >0 numerical info, <0 error), after
+ * waiting for the library stream; clears it. */
+int lila_get_info(void);
+/* Wait for the library stream; returns the status like lila_get_info. */
+int lila_sync(void);
+/* Library stream (a hipStream_t); NULL selects the default stream. */
+void lila_set_stream(void *hip_stream);
+void *lila_get_stream(void);
+
+/* Least squares through dgeqr3, Q^T B from (V,T) and the R solve, the
+ * sequence of least-squares/main_ls_qr3.c:94-119.  A (m x n) is overwritten
+ * by V and R, T (n x n) by the T factor, B (m x k) by Q^T B with X = its
+ * first n rows. */
+int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb);
+
+/* C = alpha op(A) op(B) + beta C on FP64 MFMA, dgemm argument order
+ * (transa/transb 'N' or 'T'); the GEMM every BLAS-3 step above runs on. */
+int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double *A,
+               int lda, const double *B, int ldb, double beta, double *C, int ldc);
+
+/* Device generators of the test inputs (splitmix64 stream; Box-Muller). */
+int lila_fill_uniform(unsigned long long seed, int m, int n, double *A, int lda, double lo,
+                      double hi);
+int lila_fill_normal(unsigned long long seed, int m, int n, double *A, int lda);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,458 @@
+/*
+ * api.c -- the exported C ABI (include/lila.h) of liblila_hip.so.
+ *
+ * Each entry point classifies its matrix pointers: device pointers are used
+ * in place and the call returns after enqueueing (asynchronous, like a BLAS
+ * call); host pointers are staged through device buffers covering exactly
+ * the span the reference routine touches ((cols-1)*ld + rows doubles) and
+ * copied back before the call returns.  The work of the computation is the
+ * device-pointer driver of the same name in drivers.c.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/lila.h"
+#include "lila_dev.h"
+#include "lila_drivers.h"
+
+typedef struct {
+    double *user;
+    double *dev;
+    size_t count;
+    int staged;
+} stage_t;
+
+static int g_host_staged = 0;
+
+static stage_t stage_in(double *p, int ld, int rows, int cols) {
+    stage_t s = {p, p, 0, 0};
+    if (!p || rows <= 0 || cols <= 0) return s;
+    if (ld_is_device_ptr(p)) return s;
+    s.count = (size_t)(cols - 1) * (size_t)ld + (size_t)rows;
+    s.dev = (double *)ld_dev_malloc(s.count * sizeof(double));
+    if (!s.dev) { s.dev = NULL; return s; }
+    ld_memcpy_h2d(s.dev, p, s.count * sizeof(double));
+    s.staged = 1;
+    g_host_staged = 1;
+    return s;
+}
+
+static void stage_out(stage_t *s, int copy_back) {
+    if (!s->staged) return;
+    if (copy_back) ld_memcpy_d2h(s->user, s->dev, s->count * sizeof(double));
+    ld_dev_free(s->dev);
+    s->staged = 0;
+}
+
+static int api_begin(void) {
+    if (ld_init() != 0) {
+        fprintf(stderr, "lila_hip: no usable HIP device; this library has no CPU path\n");
+        return -1;
+    }
+    g_host_staged = 0;
+    return 0;
+}
+
+/* for host-staged calls wait and report; device calls stay asynchronous */
+static int api_end(void) {
+    int r;
+    if (g_host_staged) {
+        ld_sync();
+        r = ld_get_info();
+        ld_clear_info();
+    } else {
+        /* asynchronous call: device-side status arrives with lila_get_info */
+        r = ld_get_info();
+        if (r < 0) ld_clear_info(); else r = 0;
+    }
+    if (ld_ws_mark() == 0) ld_ws_consolidate();
+    return r;
+}
+
+#define BEGIN() do { if (api_begin() != 0) return -1; } while (0)
+
+/* ---- factorization drivers --------------------------------------------- */
+
+int dgeqr3(int m, int n, double *A, int lda, double *T, int ldt) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n);
+    dv_dgeqr3(m, n, a.dev, lda, t.dev, ldt);
+    stage_out(&a, 1); stage_out(&t, 1);
+    return api_end();
+}
+
+int lila_dgeqrf_w03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *Q, int ldq, double *work, int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
+    stage_t q = stage_in(Q, ldq, m, i + n);
+    dv_dgeqrf_w03_recursive(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
+    return api_end();
+}
+
+int lila_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n, int i,
+                           int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                           double *work, int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
+    stage_t q = stage_in(Q, ldq, m, i + n);
+    dv_dgeqrf_w03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev,
+                         ldq);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
+    return api_end();
+}
+
+#define LEAF_ENTRY(NAME, CALL)                                                              \
+    int NAME {                                                                              \
+        (void)work; (void)lwork;                                                            \
+        BEGIN();                                                                            \
+        stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);           \
+        stage_t q = stage_in(Q, ldq, m, i + n);                                             \
+        CALL;                                                                               \
+        stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);                               \
+        return api_end();                                                                   \
+    }
+
+LEAF_ENTRY(lila_dgeqr2_w03(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                           int ldt, double *Q, int ldq, double *work, int lwork),
+           dv_dgeqr2_w03(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+LEAF_ENTRY(lila_dgeqrf_w03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *Q, int ldq, double *work, int lwork),
+           dv_dgeqrf_w03_mt(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+LEAF_ENTRY(lila_dgeqrf_w03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                 double *T, int ldt, double *Q, int ldq, double *work, int lwork),
+           dv_dgeqrf_w03_mt_hh(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+LEAF_ENTRY(lila_dgeqrf_w03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                                double *Q, int ldq, double *work, int lwork),
+           dv_dgeqrf_w03_mt_l(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+LEAF_ENTRY(lila_dgeqrf_w03_mt_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                                 double *Q, int ldq, double *work, int lwork),
+           dv_dgeqrf_w03_mt_hr(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+LEAF_ENTRY(lila_dgeqr2_w03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                             double *Q, int ldq, double *work, int lwork),
+           dv_dgeqr2_w03_l(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+LEAF_ENTRY(lila_dgeqr2_w03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                             double *Q, int ldq, double *work, int lwork),
+           dv_dgeqr2_w03_3(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+LEAF_ENTRY(lila_dgeqr2_w03_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                              double *Q, int ldq, double *work, int lwork),
+           dv_dgeqr2_w03_hr(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+
+/* ---- block reflector application / Q build / T ------------------------- */
+
+int lila_dormqrf_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                     int ldt, double *work, int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    int cols = (i + k > j + n) ? i + k : j + n;
+    stage_t a = stage_in(A, lda, m, cols), t = stage_in(T, ldt, ldt, i + k);
+    dv_dormqrf_w03(m, n, k, i, j, mt, a.dev, lda, t.dev, ldt);
+    stage_out(&a, 1); stage_out(&t, 0);
+    return api_end();
+}
+
+int lila_dormqrf_z03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                     int ldt, double *B, int ldb, double *work, int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + k), t = stage_in(T, ldt, ldt, i + k);
+    stage_t b = stage_in(B, ldb, m, j + n);
+    dv_dormqrf_z03(m, n, k, i, j, mt, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    stage_out(&a, 0); stage_out(&t, 0); stage_out(&b, 1);
+    return api_end();
+}
+
+int lila_dormqrbz_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *Q,
+                      int ldq, double *T, int ldt, double *work, int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + k), t = stage_in(T, ldt, ldt, i + k);
+    stage_t q = stage_in(Q, ldq, m, j + n);
+    dv_dormqrbz_w03(m, n, k, i, j, mt, a.dev, lda, q.dev, ldq, t.dev, ldt);
+    stage_out(&a, 0); stage_out(&t, 0); stage_out(&q, 1);
+    return api_end();
+}
+
+int lila_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int lda, double *T,
+                            int ldt) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
+    dv_dlarft_connect_w03(m, n, i, j, mt, a.dev, lda, t.dev, ldt);
+    stage_out(&a, 0); stage_out(&t, 1);
+    return api_end();
+}
+
+int lila_dlarft_w03(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                    double *tau) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
+    stage_t ta = stage_in(tau, n, n, 1);
+    dv_dlarft_w03(m, n, i, mt, a.dev, lda, t.dev, ldt, ta.dev);
+    stage_out(&a, 0); stage_out(&t, 1); stage_out(&ta, 0);
+    return api_end();
+}
+
+int lila_dT2tau_w03(int m, int n, int i, int mt, double *T, int ldt, double *tau) {
+    (void)m;
+    BEGIN();
+    stage_t t = stage_in(T, ldt, ldt, i + n), ta = stage_in(tau, n, n, 1);
+    ld_T2tau(n, i, mt, t.dev, ldt, ta.dev);
+    stage_out(&t, 0); stage_out(&ta, 1);
+    return api_end();
+}
+
+int lila_dV2tau_w03(int m, int n, int i, int mt, double *A, int lda, double *tau) {
+    (void)mt;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), ta = stage_in(tau, n, n, 1);
+    ld_V2tau(m, n, i, a.dev, lda, ta.dev);
+    stage_out(&a, 0); stage_out(&ta, 1);
+    return api_end();
+}
+
+/* ---- Householder reconstruction ----------------------------------------- */
+
+int lila_dorgh2_3(int m, int n, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                  double *work, int lwork, double *S) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n), q = stage_in(Q, ldq, m, n);
+    stage_t s = stage_in(S, n, n, 1);
+    dv_dorgh2_3(m, n, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1); stage_out(&s, 1);
+    return api_end();
+}
+
+int lila_dorgh2(int m, int n, int i, int j, int mt, double *A, int lda, double *T, int ldt,
+                double *Q, int ldq, double *work, int lwork, double *S) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, j + n), t = stage_in(T, ldt, ldt, j + n);
+    stage_t q = stage_in(Q, ldq, m, j + n), s = stage_in(S, j + n, j + n, 1);
+    dv_dorgh2(m, n, i, j, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1); stage_out(&s, 1);
+    return api_end();
+}
+
+int lila_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int lda, double *T,
+                       int ldt, double *Q, int ldq, double *work, int lwork, double *S) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, j + n), t = stage_in(T, ldt, ldt, j + n);
+    stage_t q = stage_in(Q, ldq, m, j + n), s = stage_in(S, j + n, j + n, 1);
+    dv_ormhr2_w03_hr(m, n, i, j, l, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1); stage_out(&s, 1);
+    return api_end();
+}
+
+int lila_dorghr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                int ldq, double *work, int lwork, int *S) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
+    stage_t q = stage_in(Q, ldq, m, i + n);
+    size_t mark = ld_ws_mark();
+    double *Sd = (double *)ld_ws_alloc(sizeof(double) * (size_t)(i + n));
+    dv_dorghr(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, Sd);
+    if (S) {
+        double *hs = (double *)malloc(sizeof(double) * (size_t)(i + n));
+        ld_memcpy_d2h(hs, Sd, sizeof(double) * (size_t)(i + n));
+        int dev_s = ld_is_device_ptr(S);
+        int *hi = dev_s ? (int *)malloc(sizeof(int) * (size_t)(i + n)) : S;
+        for (int k = i; k < i + n; k++) hi[k] = (int)hs[k];
+        if (dev_s) {
+            ld_memcpy_h2d(S + i, hi + i, sizeof(int) * (size_t)n);
+            free(hi);
+        }
+        free(hs);
+    }
+    ld_ws_release(mark);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
+    return api_end();
+}
+
+int lila_dgetrf_b(int m, int n, int i, int mt, double *A, int lda, double *Q, int ldq, double *work,
+                  int lwork) {
+    (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), q = stage_in(Q, ldq, m, i + n);
+    stage_t u = stage_in(work, n, n, n);
+    dv_dgetrf_b(m, n, i, mt, a.dev, lda, q.dev, ldq, u.dev);
+    stage_out(&a, 1); stage_out(&q, 1); stage_out(&u, 1);
+    return api_end();
+}
+
+/* ---- least squares, generators, status --------------------------------- */
+
+int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n), b = stage_in(B, ldb, m, k);
+    dv_ls_qr3(m, n, k, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&b, 1);
+    return api_end();
+}
+
+int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double *A,
+               int lda, const double *B, int ldb, double beta, double *C, int ldc) {
+    BEGIN();
+    int ta = (transa == 'T' || transa == 't'), tb = (transb == 'T' || transb == 't');
+    stage_t a = stage_in((double *)A, lda, ta ? k : m, ta ? m : k);
+    stage_t b = stage_in((double *)B, ldb, tb ? n : k, tb ? k : n);
+    stage_t c = stage_in(C, ldc, m, n);
+    ld_gemm_simple(ta, tb, m, n, k, alpha, a.dev, lda, LD_GEN, b.dev, ldb, LD_GEN, beta, c.dev, ldc);
+    stage_out(&a, 0); stage_out(&b, 0); stage_out(&c, 1);
+    return api_end();
+}
+
+int lila_fill_uniform(unsigned long long seed, int m, int n, double *A, int lda, double lo, double hi) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n);
+    ld_fill_uniform(seed, m, n, a.dev, lda, lo, hi);
+    stage_out(&a, 1);
+    return api_end();
+}
+
+int lila_fill_normal(unsigned long long seed, int m, int n, double *A, int lda) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n);
+    ld_fill_normal(seed, m, n, a.dev, lda);
+    stage_out(&a, 1);
+    return api_end();
+}
+
+int lila_get_info(void) {
+    if (ld_init() != 0) return -1;
+    ld_sync();
+    int r = ld_get_info();
+    ld_clear_info();
+    return r;
+}
+
+int lila_sync(void) {
+    if (ld_init() != 0) return -1;
+    ld_sync();
+    return ld_get_info();
+}
+
+void lila_set_stream(void *s) { ld_set_stream(s); }
+void *lila_get_stream(void) { return ld_stream(); }
+
+/* ---- workspace queries (closed forms of lila-c/lila_query_*.c) ---------- */
+
+int lila_query_dormqrf_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                           int ldt, double *work, int lwork) {
+    (void)j; (void)A; (void)lda; (void)T; (void)ldt; (void)work; (void)lwork;
+    return dq_dormqrf_w03(m, n, k, i, mt);
+}
+
+int lila_query_dormqrbz_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *Q,
+                            int ldq, double *T, int ldt, double *work, int lwork) {
+    (void)j; (void)A; (void)lda; (void)Q; (void)ldq; (void)T; (void)ldt; (void)work; (void)lwork;
+    return dq_dormqrbz_w03(m, n, k, i, mt);
+}
+
+int lila_query_dgeqrf_w03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                             double *T, int ldt, double *Q, int ldq, double *work, int lwork) {
+    (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq; (void)work;
+    return dq_dgeqrf_w03_mt(lila_param, m, n, i, mt, lwork);
+}
+
+int lila_query_dgeqrf_w03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                double *T, int ldt, double *Q, int ldq, double *work, int lwork) {
+    (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq; (void)work;
+    int p[4] = {lila_param[0], 1, lila_param[2], lila_param[3]};
+    return dq_dgeqrf_w03_mt(p, m, n, i, mt, lwork);
+}
+
+int lila_query_dgeqrf_w03_mt_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                                double *Q, int ldq, double *work, int lwork) {
+    (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq; (void)work;
+    int p[4] = {0, 2, 0, 0};
+    return dq_dgeqrf_w03_mt(p, m, n, i, mt, lwork);
+}
+
+int lila_query_dgeqrf_w03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                               double *Q, int ldq, double *work, int lwork) {
+    (void)m; (void)i; (void)mt; (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq;
+    (void)work; (void)lwork;
+    return n * n;
+}
+
+int lila_query_dgeqr2_w03_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                             double *Q, int ldq, double *work, int lwork) {
+    (void)m; (void)i; (void)mt; (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq;
+    (void)work; (void)lwork;
+    return n;
+}
+
+int lila_query_dgeqr2_w03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                            double *Q, int ldq, double *work, int lwork) {
+    (void)m; (void)i; (void)mt; (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq;
+    (void)work; (void)lwork;
+    return n * n;
+}
+
+int lila_query_dgeqr2_w03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                            double *Q, int ldq, double *work, int lwork) {
+    (void)m; (void)i; (void)mt; (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq;
+    (void)work; (void)lwork;
+    return n * n;
+}
+
+int lila_query_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int lda, double *T,
+                             int ldt, double *Q, int ldq, double *work, int lwork, double *S) {
+    (void)m; (void)l; (void)mt; (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq;
+    (void)work; (void)lwork; (void)S;
+    return n * (j - i) + n;
+}
+
+int lila_query_dorgh2_3(int m, int n, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                        double *work, int lwork, double *S) {
+    (void)m; (void)n; (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq; (void)work;
+    (void)lwork; (void)S;
+    return 0;
+}
+
+int lila_query_dgeqrf_w03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                    double *T, int ldt, double *Q, int ldq, double *work, int lwork) {
+    (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq; (void)work;
+    return dq_dgeqrf_w03_recursive(lila_param, m, n, i, mt, lwork);
+}
+
+int lila_query_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n,
+                                 int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                                 int ldq, double *work, int lwork) {
+    (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq; (void)work; (void)lwork;
+    return dq_dgeqrf_w03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt);
+}
+
+/* lila-c/lila_query_dgeqrf_ker_recursive.c: the recursive results are
+ * overwritten before use there, so only the leaf (n+i, n*n) or split
+ * (nb1*nb2) terms of the top call count */
+int lila_query_dgeqrf_ker_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                    double *T, int ldt, double *Q, int ldq, double *work, int lwork) {
+    (void)m; (void)mt; (void)A; (void)lda; (void)T; (void)ldt; (void)Q; (void)ldq; (void)work;
+    int w = lwork;
+    if (n <= lila_param[3]) {
+        if (w < n + i) w = n + i;
+        if (w < n * n) w = n * n;
+    } else {
+        int nb1 = n / 2, nb2 = n - nb1;
+        if (w < nb1 * nb2) w = nb1 * nb2;
+    }
+    return w;
+}
+
+/* lila-c/lila_query_dgeqrf_LAPACK_appendcols.c */
+int lila_query_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A, int lda, double *Q,
+                                        int ldq, double *T, int ldt, double *work, int lwork) {
+    (void)m; (void)mt; (void)A; (void)lda; (void)Q; (void)ldq; (void)T; (void)ldt; (void)work;
+    int w = lwork;
+    if (w < n * k) w = n * k;
+    if (w < n * n) w = n * n;
+    return w;
+}

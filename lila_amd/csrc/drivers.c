@@ -1,0 +1,541 @@
+/*
+ * drivers.c -- host-side orchestration of the LILA hot path on one MI355X.
+ *
+ * Plain C.  Each dv_* routine keeps the call tree and the global-index /
+ * mt-blocked-T pointer arithmetic of the reference routine it replaces
+ * (cited per function), and issues device work through lila_dev.h: the
+ * BLAS-3 steps become FP64 MFMA GEMM launches that read the Householder V
+ * (unit lower) and T (upper) in place through operand structure masks, the
+ * panel becomes one fused launch, and the Cholesky-QR / reconstruction
+ * steps become TRSM / SYRK / Cholesky / sign-LU launches.  All pointers are
+ * device pointers; all work goes onto the library stream.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "lila_dev.h"
+#include "lila_drivers.h"
+
+#define N_ 0
+#define T_ 1
+
+static double *wsd(size_t n) { return (double *)ld_ws_alloc(n * sizeof(double)); }
+
+static void gemm(int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
+                 int sa, const double *B, int ldb, int sb, double beta, double *C, int ldc) {
+    ld_gemm_simple(ta, tb, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc);
+}
+
+/* ---- block reflector pieces ------------------------------------------- */
+
+/* C (rows [0, ml) of the block) <- (I - V T V^T)^T C for one block of vb
+ * reflectors: W = V^T C; W = T^T W; C -= V W. */
+static void apply_block_qt(int ml, int n, int vb, const double *V, int ldv, const double *Tb,
+                           int ldt, double *C, int ldc) {
+    if (vb <= 0 || n <= 0) return;
+    size_t mark = ld_ws_mark();
+    double *W1 = wsd((size_t)vb * n), *W2 = wsd((size_t)vb * n);
+    gemm(T_, N_, vb, n, ml, 1.0, V, ldv, LD_VUNIT, C, ldc, LD_GEN, 0.0, W1, vb);
+    gemm(T_, N_, vb, n, vb, 1.0, Tb, ldt, LD_UPPER, W1, vb, LD_GEN, 0.0, W2, vb);
+    gemm(N_, N_, ml, n, vb, -1.0, V, ldv, LD_VUNIT, W2, vb, LD_GEN, 1.0, C, ldc);
+    ld_ws_release(mark);
+}
+
+/* C <- H^T C for the k reflectors starting at global i, mt-block by
+ * mt-block; C row 0 is global row i.  Body shared by
+ * lila-c/lila_dormqrf_w03.c:3-81 and lila-c/lila_dormqrf_z03.c:3-72: block b
+ * starts at global b0, has vb = min(mt - b0%mt, remaining) reflectors and
+ * its T at T + b0%mt + b0*ldt (the walk of lila-c/lila_dormqrf_w03.c:63,71). */
+static void apply_qt(int m, int n, int k, int i, int mt, const double *A, int lda,
+                     const double *T, int ldt, double *C, int ldc) {
+    if (k <= 0 || n <= 0) return;
+    int b0 = i, done = 0, vb = mt - i % mt;
+    if (vb > k) vb = k;
+    while (done < k) {
+        apply_block_qt(m - b0, n, vb, A + b0 + (size_t)b0 * lda, lda, T + b0 % mt + (size_t)b0 * ldt,
+                       ldt, C + (b0 - i), ldc);
+        done += vb;
+        b0 += vb;
+        vb = (k - done >= mt) ? mt : (k - done);
+    }
+}
+
+int dv_dormqrf_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T, int ldt) {
+    apply_qt(m, n, k, i, mt, A, lda, T, ldt, A + i + (size_t)j * lda, lda);
+    return 0;
+}
+
+int dv_dormqrf_z03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T, int ldt,
+                   double *B, int ldb) {
+    apply_qt(m, n, k, i, mt, A, lda, T, ldt, B + i + (size_t)j * ldb, ldb);
+    return 0;
+}
+
+/* Q(:, j:j+n) <- H_i ... H_{i+k-1} [0; Q_new], mt-blocks walked backwards;
+ * lila-c/lila_dormqrbz_w03.c:3-66.  Per block: W = V2^T Q2; W = T W;
+ * Q_block = [0; Q2] - V W (one GEMM; the top vb rows take beta = 0, which is
+ * the dlacpy + trmm of :27-28). */
+int dv_dormqrbz_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *Q, int ldq,
+                    double *T, int ldt) {
+    if (k <= 0 || n <= 0) return 0;
+    int cap = k / (mt > 0 ? mt : 1) + 2;
+    int *start = (int *)malloc((size_t)cap * sizeof(int));
+    int nblk = 0, b0 = i, vb = mt - i % mt;
+    if (vb > k) vb = k;
+    while (b0 < i + k) {
+        start[nblk++] = b0;
+        b0 += vb;
+        vb = (i + k - b0 >= mt) ? mt : (i + k - b0);
+    }
+    for (int b = nblk - 1; b >= 0; b--) {
+        int s = start[b], e = (b + 1 < nblk) ? start[b + 1] : i + k;
+        int w = e - s, ml = m - s;
+        const double *Vb = A + s + (size_t)s * lda;
+        const double *Tb = T + s % mt + (size_t)s * ldt;
+        double *Qb = Q + s + (size_t)j * ldq;
+        size_t mark = ld_ws_mark();
+        double *W1 = wsd((size_t)w * n), *W2 = wsd((size_t)w * n);
+        gemm(T_, N_, w, n, ml - w, 1.0, Vb + w, lda, LD_GEN, Qb + w, ldq, LD_GEN, 0.0, W1, w);
+        gemm(N_, N_, w, n, w, 1.0, Tb, ldt, LD_UPPER, W1, w, LD_GEN, 0.0, W2, w);
+        ld_gemm_t g;
+        memset(&g, 0, sizeof g);
+        g.m = ml; g.n = n; g.k = w; g.alpha = -1.0; g.beta = 1.0;
+        g.A = Vb; g.lda = lda; g.ta = 0; g.sa = LD_VUNIT;
+        g.B = W2; g.ldb = w; g.tb = 0; g.sb = LD_GEN;
+        g.C = Qb; g.ldc = ldq; g.zero_rows = w;
+        ld_gemm(&g);
+        ld_ws_release(mark);
+    }
+    free(start);
+    return 0;
+}
+
+/* Join T of the reflectors [i, i+vb) with the wb earlier ones of the same
+ * mt-block: T_ji = -T_jj (V_j^T V_i) T_ii; lila-c/lila_dlarft_connect_w03.c:3-32
+ * (the wb/vb rule of :8-11; the copy + trmm + gemm of :19-26 is one GEMM with
+ * V_i read as unit lower). */
+int dv_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int lda, double *T, int ldt) {
+    int vb = mt - i % mt;
+    if (vb > n) vb = n;
+    int wb = ((j / mt) == (i / mt)) ? (i - j) : (i % mt);
+    if (wb == mt) wb = 1;
+    if (wb <= 0 || vb <= 0) return 0;
+    const double *Aii = A + i + (size_t)i * lda;
+    const double *Aij = A + i + (size_t)(i - wb) * lda;
+    const double *Tki = T + i % mt + (size_t)i * ldt;
+    double *Tji = T + (i % mt - wb) + (size_t)i * ldt;
+    const double *Tjj = T + (i % mt - wb) + (size_t)(i - wb) * ldt;
+    size_t mark = ld_ws_mark();
+    double *G = wsd((size_t)wb * vb), *H = wsd((size_t)wb * vb);
+    gemm(T_, N_, wb, vb, m - i, 1.0, Aij, lda, LD_GEN, Aii, lda, LD_VUNIT, 0.0, G, wb);
+    gemm(N_, N_, wb, vb, wb, 1.0, Tjj, ldt, LD_UPPER, G, wb, LD_GEN, 0.0, H, wb);
+    gemm(N_, N_, wb, vb, vb, -1.0, H, wb, LD_GEN, Tki, ldt, LD_UPPER, 0.0, Tji, ldt);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* ---- panels ------------------------------------------------------------- */
+
+/* Recursive panel QR (Elmroth-Gustavson), lila-c/dgeqr3.c:3-114: the left
+ * half recursively, its reflectors applied to the right half (:44-70), the
+ * right half recursively (:75), T12 = -T11 (V1^T V2) T22 (:95-108).  Widths
+ * up to 32 go to the fused panel kernel. */
+int dv_dgeqr3(int m, int n, double *A, int lda, double *T, int ldt) {
+    if (n <= 0) return 0;
+    if (n <= LD_PANEL_MAX) {
+        ld_panel_qr(m, n, A, lda, T, ldt, NULL, 0);
+        return 0;
+    }
+    int n1 = n / 2, n2 = n - n1;
+    dv_dgeqr3(m, n1, A, lda, T, ldt);
+    apply_block_qt(m, n2, n1, A, lda, T, ldt, A + (size_t)n1 * lda, lda);
+    dv_dgeqr3(m - n1, n2, A + n1 + (size_t)n1 * lda, lda, T + n1 + (size_t)n1 * ldt, ldt);
+    size_t mark = ld_ws_mark();
+    double *G = wsd((size_t)n1 * n2), *H = wsd((size_t)n1 * n2);
+    gemm(T_, N_, n1, n2, m - n1, 1.0, A + n1, lda, LD_GEN, A + n1 + (size_t)n1 * lda, lda, LD_VUNIT,
+         0.0, G, n1);
+    gemm(N_, N_, n1, n2, n1, 1.0, T, ldt, LD_UPPER, G, n1, LD_GEN, 0.0, H, n1);
+    gemm(N_, N_, n1, n2, n2, -1.0, H, n1, LD_GEN, T + n1 + (size_t)n1 * ldt, ldt, LD_UPPER, 0.0,
+         T + (size_t)n1 * ldt, ldt);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* explicit panel Q = [I;0] - V (T V1^T) (lila-c/lila_dgeqr2_w03_3.c:19-30) */
+static void panel_q(int ml, int n, const double *V, int ldv, const double *T, int ldt, double *Q,
+                    int ldq) {
+    size_t mark = ld_ws_mark();
+    double *X = wsd((size_t)n * n);
+    gemm(N_, T_, n, n, n, 1.0, T, ldt, LD_UPPER, V, ldv, LD_VUNIT, 0.0, X, n);
+    gemm(N_, N_, ml, n, n, -1.0, V, ldv, LD_VUNIT, X, n, LD_UPPER, 0.0, Q, ldq);
+    ld_add_diag(n, Q, ldq, 1.0);
+    ld_ws_release(mark);
+}
+
+/* dgeqr3 + explicit panel Q; lila-c/lila_dgeqr2_w03_3.c:3-34 */
+int dv_dgeqr2_w03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                    int ldq) {
+    int ml = m - i;
+    double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
+    double *Tki = T + i % mt + (size_t)i * ldt;
+    if (n <= LD_PANEL_MAX) {
+        ld_panel_qr(ml, n, Aii, lda, Tki, ldt, Qii, ldq);
+    } else {
+        dv_dgeqr3(ml, n, Aii, lda, Tki, ldt);
+        panel_q(ml, n, Aii, lda, Tki, ldt, Qii, ldq);
+    }
+    return 0;
+}
+
+/* LAPACK-style panel (dgeqrf + dlarft + dorgqr; lila-c/lila_dgeqr2_w03_l.c).
+ * On the GPU the Householder vectors, tau and T of dgeqrf + dlarft are
+ * exactly what the recursive panel produces, so it shares that path. */
+int dv_dgeqr2_w03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                    int ldq) {
+    return dv_dgeqr2_w03_3(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+}
+
+/* Sign-chosen LU reconstruction of one block (lila-c/lila_dorgh2_3.c:8-51):
+ * A (ml rows) holds R on top and Q2 below, Tq the n x n Q1 copy, Q the
+ * explicit Q (mq rows) whose columns are flipped; S (device) gets +-1. */
+static void reconstruct_block(int ml, int n, double *A, int lda, double *Tq, int ldt, double *Q,
+                              int ldq, int mq, double *S) {
+    ld_lu_sign(n, Tq, ldt, S);
+    if (ml > n) {
+        ld_flip_cols(ml - n, n, A + n, lda, S);
+        ld_trsm('R', 'U', 'N', 'N', ml - n, n, 1.0, Tq, ldt, A + n, lda);
+    }
+    ld_flip_rows_upper(n, A, lda, S);
+    ld_flip_cols(mq, n, Q, ldq, S);
+    ld_move_lower(n, Tq, ldt, A, lda);
+    ld_trsm('R', 'L', 'T', 'U', n, n, -1.0, A, lda, Tq, ldt);
+}
+
+int dv_dorgh2_3(int m, int n, double *A, int lda, double *T, int ldt, double *Q, int ldq, double *S) {
+    reconstruct_block(m, n, A, lda, T, ldt, Q, ldq, m, S);
+    return 0;
+}
+
+/* global-index form, mt layout (storage/c2/lila_dorgh2_w03.c:3-65) */
+int dv_dorgh2(int m, int n, int i, int j, int mt, double *A, int lda, double *T, int ldt, double *Q,
+              int ldq, double *S) {
+    reconstruct_block(m - j, n, A + j + (size_t)j * lda, lda, T + j % mt + (size_t)j * ldt, ldt,
+                      Q + i + (size_t)j * ldq, ldq, m - i, S + j);
+    return 0;
+}
+
+/* blocked reconstruction step, lila-c/lila_ormhr2_w03_hr.c:3-84 (S kept
+ * apart from the workspace) */
+int dv_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int lda, double *T, int ldt,
+                     double *Q, int ldq, double *S) {
+    (void)l;
+    int jb = j - i;
+    double *Tjj = T + j % mt + (size_t)j * ldt;
+    double *Ajj = A + j + (size_t)j * lda;
+    size_t mark = ld_ws_mark();
+    if (jb > 0) ld_flip_rows(jb, n, A + i + (size_t)j * lda, lda, S + i);
+    double *W2 = wsd((size_t)(jb > 0 ? jb : 1) * n);
+    if (jb > 0) ld_lacpy('A', jb, n, Q + i + (size_t)j * ldq, ldq, W2, jb);
+    ld_lacpy('A', n, n, Q + j + (size_t)j * ldq, ldq, Tjj, ldt);
+    ld_lacpy('A', m - j - n, n, Q + j + n + (size_t)j * ldq, ldq, Ajj + n, lda);
+    if (jb > 0) {
+        ld_trsm('L', 'L', 'N', 'U', jb, n, 1.0, A + i + (size_t)i * lda, lda, W2, jb);
+        gemm(N_, N_, n, n, jb, -1.0, A + j + (size_t)i * lda, lda, LD_GEN, W2, jb, LD_GEN, 1.0, Tjj, ldt);
+        gemm(N_, N_, m - j - n, n, jb, -1.0, A + j + n + (size_t)i * lda, lda, LD_GEN, W2, jb, LD_GEN,
+             1.0, Ajj + n, lda);
+    }
+    reconstruct_block(m - j, n, Ajj, lda, Tjj, ldt, Q + i + (size_t)j * ldq, ldq, m - i, S + j);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* blocked reconstruction driver (loop of lila-c/lila_dgeqrf_w03_mt_hr.c:23-29);
+ * Sd: device doubles indexed by global column */
+int dv_dorghr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+              double *Sd) {
+    int j = i, vb = mt - i % mt;
+    if (vb > n) vb = n;
+    while (vb > 0) {
+        dv_ormhr2_w03_hr(m, vb, i, j, -1, mt, A, lda, T, ldt, Q, ldq, Sd);
+        j += vb;
+        vb = (j + mt >= i + n) ? (i + n - j) : mt;
+    }
+    return 0;
+}
+
+/* one Cholesky-QR pass: Q = A R^{-1}, R = chol(A^T A) in the top of A */
+static void cholqr(int ml, int n, double *Aii, int lda, double *Qii, int ldq) {
+    ld_lacpy('A', ml, n, Aii, lda, Qii, ldq);
+    ld_syrk_ut(n, ml, 1.0, Qii, ldq, 0.0, Aii, lda);
+    ld_potrf_u(n, Aii, lda);
+    ld_trsm('R', 'U', 'N', 'N', ml, n, 1.0, Aii, lda, Qii, ldq);
+}
+
+/* CholQR leaf + blocked reconstruction; lila-c/lila_dgeqrf_w03_mt_hr.c:3-33 */
+int dv_dgeqrf_w03_mt_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                        int ldq) {
+    int ml = m - i;
+    cholqr(ml, n, A + i + (size_t)i * lda, lda, Q + i + (size_t)i * ldq, ldq);
+    size_t mark = ld_ws_mark();
+    double *Sd = wsd((size_t)(i + n));
+    dv_dorghr(m, n, i, mt, A, lda, T, ldt, Q, ldq, Sd);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* CholQR panel + dorgh2_3; lila-c/lila_dgeqr2_w03_hr.c:3-30 */
+int dv_dgeqr2_w03_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                     int ldq) {
+    int ml = m - i;
+    double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
+    double *Tki = T + i % mt + (size_t)i * ldt;
+    cholqr(ml, n, Aii, lda, Qii, ldq);
+    ld_lacpy('A', ml - n, n, Qii + n, ldq, Aii + n, lda);
+    ld_lacpy('A', n, n, Qii, ldq, Tki, ldt);
+    size_t mark = ld_ws_mark();
+    double *S = wsd((size_t)n);
+    dv_dorgh2_3(ml, n, Aii, lda, Tki, ldt, Qii, ldq, S);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* panel dispatch on lila_param[2]; lila-c/lila_dgeqr2_w03.c:8-16 */
+int dv_dgeqr2_w03(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                  double *Q, int ldq) {
+    switch (p[2]) {
+    case 0: return dv_dgeqr2_w03_l(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    case 1: return dv_dgeqr2_w03_3(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    case 2: return dv_dgeqr2_w03_hr(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    }
+    return 0;
+}
+
+/* per-mt-block T from V and tau (lila-c/lila_dlarft_w03.c:3-55) */
+int dv_dlarft_w03(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, const double *tau) {
+    int j = i, vb = mt - i % mt;
+    if (vb > n) vb = n;
+    while (vb > 0) {
+        ld_larft(m - j, vb, A + j + (size_t)j * lda, lda, tau + (j - i), T + j % mt + (size_t)j * ldt, ldt);
+        j += vb;
+        vb = (j + mt >= i + n) ? (i + n - j) : mt;
+    }
+    return 0;
+}
+
+/* LAPACK leaf (dgeqrf, lila_dlarft_w03, dorgqr; lila-c/lila_dgeqrf_w03_mt_l.c):
+ * the recursive panel gives V and the full T (whose diagonal is tau); T is
+ * then laid out per mt-block by dlarft_w03 and Q = [I;0] - V T V1^T. */
+int dv_dgeqrf_w03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                       int ldq) {
+    int ml = m - i;
+    double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
+    size_t mark = ld_ws_mark();
+    double *Tf = wsd((size_t)n * n), *tau = wsd((size_t)n);
+    ld_laset('L', n, n, 0.0, 0.0, Tf, n);
+    dv_dgeqr3(ml, n, Aii, lda, Tf, n);
+    ld_T2tau(n, 0, n + 1, Tf, n, tau);
+    dv_dlarft_w03(m, n, i, mt, A, lda, T, ldt, tau);
+    panel_q(ml, n, Aii, lda, Tf, n, Qii, ldq);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* hh leaf; lila-c/lila_dgeqrf_w03_mt_hh.c:7-32 */
+int dv_dgeqrf_w03_mt_hh(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                        int ldt, double *Q, int ldq) {
+    int vb = mt - i % mt;
+    if (vb > n) vb = n;
+    dv_dgeqr2_w03(p, m, vb, i, mt, A, lda, T, ldt, Q, ldq);
+    int j = i + vb, l = vb;
+    vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    while (vb != 0) {
+        dv_dormqrf_w03(m, vb, l, i, j, mt, A, lda, T, ldt);
+        dv_dgeqr2_w03(p, m, vb, j, mt, A, lda, T, ldt, Q, ldq);
+        dv_dormqrbz_w03(m, vb, l, i, j, mt, A, lda, Q, ldq, T, ldt);
+        j += vb;
+        l += vb;
+        vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    }
+    return 0;
+}
+
+/* leaf dispatch on lila_param[1]; lila-c/lila_dgeqrf_w03_mt.c:6-16 */
+int dv_dgeqrf_w03_mt(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                     double *Q, int ldq) {
+    switch (p[1]) {
+    case 0: return dv_dgeqrf_w03_mt_l(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    case 1: return dv_dgeqrf_w03_mt_hh(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    case 2: return dv_dgeqrf_w03_mt_hr(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    }
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_w03_recursive.c:3-29 */
+int dv_dgeqrf_w03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                            int ldt, double *Q, int ldq) {
+    if (n <= p[3]) return dv_dgeqrf_w03_mt(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    int n1 = n / 2, n2 = n - n1;
+    dv_dgeqrf_w03_recursive(p, m, n1, i, mt, A, lda, T, ldt, Q, ldq);
+    dv_dormqrf_w03(m, n2, n1, i, i + n1, mt, A, lda, T, ldt);
+    dv_dgeqrf_w03_recursive(p, m, n2, i + n1, mt, A, lda, T, ldt, Q, ldq);
+    dv_dlarft_connect_w03(m, n2, i + n1, i, mt, A, lda, T, ldt);
+    dv_dormqrbz_w03(m, n2, n1, i, i + n1, mt, A, lda, Q, ldq, T, ldt);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_w03_levelx.c:3-52 (i_lvl by value) */
+int dv_dgeqrf_w03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
+                         int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq) {
+    int nb = nb_lvl[i_lvl];
+    int vb = nb > n ? n : nb;
+    int k = 0, j = i;
+    if (i_lvl == n_lvl - 1) dv_dgeqrf_w03_mt(p, m, vb, i, mt, A, lda, T, ldt, Q, ldq);
+    else dv_dgeqrf_w03_levelx(p, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda, T, ldt, Q, ldq);
+    k += vb;
+    j += vb;
+    while (j < i + n) {
+        vb = (j + nb > i + n) ? (i + n - j) : nb;
+        dv_dormqrf_w03(m, vb, k, i, j, mt, A, lda, T, ldt);
+        if (i_lvl == n_lvl - 1) dv_dgeqrf_w03_mt(p, m, vb, j, mt, A, lda, T, ldt, Q, ldq);
+        else dv_dgeqrf_w03_levelx(p, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda, T, ldt, Q, ldq);
+        dv_dlarft_connect_w03(m, vb, j, i, mt, A, lda, T, ldt);
+        dv_dormqrbz_w03(m, vb, k, i, j, mt, A, lda, Q, ldq, T, ldt);
+        k += vb;
+        j += vb;
+    }
+    return 0;
+}
+
+/* c3/lila_dgetrf_b.c:3-34; U (R, n x n ld n) is the device work array */
+int dv_dgetrf_b(int m, int n, int i, int mt, double *A, int lda, double *Q, int ldq, double *U) {
+    (void)mt;
+    ld_getrf_b(m - i, n, A + i + (size_t)i * lda, lda, Q + i + (size_t)i * ldq, ldq, U);
+    return 0;
+}
+
+/* least squares through (V, T, R): lila_dgels_qr3 semantic of
+ * least-squares/main_ls_qr3.c:94-119.  B (m x k) overwritten, X in rows [0, n). */
+int dv_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb) {
+    dv_dgeqr3(m, n, A, lda, T, ldt);
+    apply_block_qt(m, k, n, A, lda, T, ldt, B, ldb);
+    ld_trsm('L', 'U', 'N', 'N', n, k, 1.0, A, lda, B, ldb);
+    return 0;
+}
+
+/* ---- workspace queries: the reference's closed forms (lila-c/lila_query_*.c) */
+
+int dq_dormqrf_w03(int m, int n, int k, int i, int mt) {
+    int vb = mt - i % mt;
+    if (vb > k) vb = k;
+    int w = n * mt, ml = m - i, jj = 1;
+    for (;;) {
+        if (w < (ml - vb) * n) w = (ml - vb) * n;
+        if (jj + vb - 1 >= k) break;
+        ml -= vb;
+        jj += vb;
+        vb = (jj + mt - 1 <= k) ? mt : (k - jj + 1);
+    }
+    return w;
+}
+
+int dq_dormqrbz_w03(int m, int n, int k, int i, int mt) {
+    int vb = (i + k) % mt;
+    if (vb == 0) vb = mt;
+    if (vb > k) vb = k;
+    int w = n * k, ml = m - (i + k - vb), jj = 1;
+    while (jj + vb - 1 < k) {
+        if (w < n * vb) w = n * vb;
+        if (w < (ml - vb) * k) w = (ml - vb) * k;
+        jj += vb;
+        vb = (jj + mt - 1 <= k) ? mt : (k - jj + 1);
+        ml += vb;
+    }
+    return w;
+}
+
+static int dq_dgeqr2(const int *p, int n) {
+    if (p[2] == 0) return n * n;
+    if (p[2] == 1) return n * n;
+    if (p[2] == 2) return n;
+    return 0;
+}
+
+int dq_dgeqrf_w03_mt(const int *p, int m, int n, int i, int mt, int lwork) {
+    int w = lwork, q;
+    if (p[1] == 0) {
+        if (w < n * n) w = n * n;
+    } else if (p[1] == 1) {
+        int vb = mt - i % mt;
+        if (vb > n) vb = n;
+        q = dq_dgeqr2(p, vb);
+        if (w < q) w = q;
+        int j = i + vb, l = vb;
+        vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+        while (vb != 0) {
+            q = dq_dormqrf_w03(m, vb, l, i, mt);
+            if (w < q) w = q;
+            q = dq_dgeqr2(p, vb);
+            if (w < q) w = q;
+            q = dq_dormqrbz_w03(m, vb, l, i, mt);
+            if (w < q) w = q;
+            j += vb;
+            l += vb;
+            vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+        }
+    } else if (p[1] == 2) {
+        int j = i, vb = mt - i % mt;
+        if (vb > n) vb = n;
+        while (vb != 0) {
+            q = vb * (j - i) + vb;
+            if (w < q) w = q;
+            j += vb;
+            vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+        }
+    }
+    return w;
+}
+
+int dq_dgeqrf_w03_recursive(const int *p, int m, int n, int i, int mt, int lwork) {
+    int w = lwork, q;
+    if (n <= p[3]) {
+        q = dq_dgeqrf_w03_mt(p, m, n, i, mt, w);
+        if (w < q) w = q;
+    } else {
+        int n1 = n / 2, n2 = n - n1;
+        q = dq_dgeqrf_w03_recursive(p, m, n1, i, mt, w);
+        if (w < q) w = q;
+        q = dq_dormqrf_w03(m, n2, n1, i, mt);
+        if (w < q) w = q;
+        q = dq_dgeqrf_w03_recursive(p, m, n2, i + n1, mt, w);
+        if (w < q) w = q;
+        q = dq_dormqrbz_w03(m, n2, n1, i, mt);
+        if (w < q) w = q;
+    }
+    return w;
+}
+
+int dq_dgeqrf_w03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
+                         int mt) {
+    int w = 0, q;
+    int nb = nb_lvl[i_lvl];
+    int vb = nb > n ? n : nb;
+    int k = 0, j = i;
+    if (i_lvl == n_lvl - 1) q = dq_dgeqrf_w03_mt(p, m, n, i, mt, w);
+    else q = dq_dgeqrf_w03_levelx(p, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt);
+    if (q > w) w = q;
+    k += vb;
+    j += vb;
+    while (j < i + n) {
+        vb = (j + nb > i + n) ? (i + n - j) : nb;
+        q = dq_dormqrf_w03(m, vb, k, i, mt);
+        if (q > w) w = q;
+        if (i_lvl == n_lvl - 1) q = dq_dgeqrf_w03_mt(p, m, n, i, mt, w);
+        else q = dq_dgeqrf_w03_levelx(p, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt);
+        if (q > w) w = q;
+        q = dq_dormqrbz_w03(m, vb, k, i, mt);
+        if (q > w) w = q;
+        k += vb;
+        j += vb;
+    }
+    return w;
+}

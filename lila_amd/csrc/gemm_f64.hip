@@ -1,0 +1,257 @@
+// gemm_f64.hip -- FP64 MFMA GEMM for the LILA hot path (gfx950).
+//
+// C = alpha * op(A) * op(B) + beta * C, column-major, with per-operand
+// structure masks so that Householder V (unit lower trapezoidal), T and R
+// (upper triangular) are read in place without being copied out:
+//   W = V^T C, W = T^T W, C -= V W (block-reflector application) and
+//   G = V_j^T V_i (T-factor connection) are all single launches.
+//
+// Tiling: 256 threads = 4 waves in a 2 x 2 grid; block tile BM x BN with
+// BK = 16; each wave owns (BM/2) x (BN/2) built from 16x16 tiles of
+// v_mfma_f64_16x16x4_f64.  The MFMA operands are fed swapped (op(B)^T as the
+// 16x4 "A" operand, op(A)^T as the 4x16 "B" operand) so the accumulator holds
+// C^T fragments: lane l owns C row (l % 16) of columns (l/16) + 4v, which
+// makes every store a run of 16 consecutive doubles of one column.
+//
+// Long-K shapes (M, N small, K = m - i in the hundreds of thousands) are
+// split over grid.z into FP64 partials that a second kernel reduces in a
+// fixed order (deterministic).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "lila_dev.h"
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BK = 16;
+constexpr int NT = 256;
+
+__device__ __forceinline__ double mask_val(const double *__restrict__ X, int ldx,
+                                           int r, int c, int s) {
+  // r, c: stored indices (already bounds-checked by the caller)
+  if (s == LD_VUNIT) {
+    if (r < c) return 0.0;
+    if (r == c) return 1.0;
+  } else if (s == LD_UPPER) {
+    if (r > c) return 0.0;
+  } else if (s == LD_LOWER) {
+    if (r < c) return 0.0;
+  } else if (s == LD_UUNIT) {
+    if (r > c) return 0.0;
+    if (r == c) return 1.0;
+  }
+  return X[(size_t)r + (size_t)c * ldx];
+}
+
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(NT) void gemm_f64_kernel(
+    int M, int N, int K, double alpha, double beta,
+    const double *__restrict__ A, int lda, int sa,
+    const double *__restrict__ B, int ldb, int sb,
+    double *__restrict__ C, int ldc, int zero_rows, int c_upper,
+    int kchunk, double *__restrict__ P) {
+  constexpr int PA = 1, PB = 1;
+  // SYRK form: a tile entirely below the diagonal writes nothing
+  if (c_upper && (int)(blockIdx.x * BM) > (int)(blockIdx.y * BN + BN - 1)) return;
+  constexpr int WMT = BM / 32;  // 16x16 tiles per wave along m
+  constexpr int WNT = BN / 32;  // along n
+  constexpr int NLA = BM * BK / NT;
+  constexpr int NLB = BN * BK / NT;
+  __shared__ double As[BK][BM + PA];
+  __shared__ double Bs[BK][BN + PB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+
+  d4_t acc[WMT][WNT];
+#pragma unroll
+  for (int a = 0; a < WMT; a++)
+#pragma unroll
+    for (int b = 0; b < WNT; b++) acc[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  double ra[NLA], rb[NLB];
+
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int t = 0; t < NLA; t++) {
+      int e = tid + t * NT;
+      int i, p;
+      if (!TA) { i = e % BM; p = e / BM; } else { p = e % BK; i = e / BK; }
+      int gi = m0 + i, gp = k0 + p;
+      double v = 0.0;
+      if (gi < M && gp < kend) v = TA ? mask_val(A, lda, gp, gi, sa) : mask_val(A, lda, gi, gp, sa);
+      ra[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < NLB; t++) {
+      int e = tid + t * NT;
+      int j, p;
+      if (!TB) { p = e % BK; j = e / BK; } else { j = e % BN; p = e / BN; }
+      int gj = n0 + j, gp = k0 + p;
+      double v = 0.0;
+      if (gj < N && gp < kend) v = TB ? mask_val(B, ldb, gj, gp, sb) : mask_val(B, ldb, gp, gj, sb);
+      rb[t] = v;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int t = 0; t < NLA; t++) {
+      int e = tid + t * NT;
+      int i, p;
+      if (!TA) { i = e % BM; p = e / BM; } else { p = e % BK; i = e / BK; }
+      As[p][i] = ra[t];
+    }
+#pragma unroll
+    for (int t = 0; t < NLB; t++) {
+      int e = tid + t * NT;
+      int j, p;
+      if (!TB) { p = e % BK; j = e / BK; } else { j = e % BN; p = e / BN; }
+      Bs[p][j] = rb[t];
+    }
+  };
+
+  if (kbeg < kend) load_tiles(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (k0 + BK < kend) load_tiles(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; kk++) {
+      const int kr = kk * 4 + (lane >> 4);
+      double af[WMT], bf[WNT];
+#pragma unroll
+      for (int a = 0; a < WMT; a++) af[a] = As[kr][wm * (BM / 2) + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int b = 0; b < WNT; b++) bf[b] = Bs[kr][wn * (BN / 2) + b * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < WMT; a++)
+#pragma unroll
+        for (int b = 0; b < WNT; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
+    }
+  }
+
+  // epilogue: the f64 16x16x4 accumulator puts D row (lane/16) + 4v in lane
+  // `lane`, register v (rows interleaved by 4), D column lane%16; D = C^T
+  // here, so lane holds C(row = base + lane%16, col = base + lane/16 + 4v)
+  const int rl = lane & 15, cg = lane >> 4;
+#pragma unroll
+  for (int a = 0; a < WMT; a++) {
+    const int gi = m0 + wm * (BM / 2) + a * 16 + rl;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int b = 0; b < WNT; b++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = n0 + wn * (BN / 2) + b * 16 + cg + 4 * v;
+        if (gj >= N || (c_upper && gi > gj)) continue;
+        double r = acc[a][b][v];
+        if (P) {
+          P[(size_t)blockIdx.z * M * N + (size_t)gi + (size_t)gj * M] = r;
+        } else {
+          double *c = &C[(size_t)gi + (size_t)gj * ldc];
+          double o = alpha * r;
+          if (beta != 0.0 && gi >= zero_rows) o += beta * (*c);
+          *c = o;
+        }
+      }
+    }
+  }
+}
+
+__global__ void splitk_reduce_kernel(int M, int N, int S, double alpha, double beta,
+                                     const double *__restrict__ P,
+                                     double *__restrict__ C, int ldc, int zero_rows,
+                                     int c_upper) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t tot = (size_t)M * N;
+  if (idx >= tot) return;
+  int i = (int)(idx % M), j = (int)(idx / M);
+  if (c_upper && i > j) return;
+  double s = 0.0;
+  for (int z = 0; z < S; z++) s += P[(size_t)z * tot + idx];
+  double *c = &C[(size_t)i + (size_t)j * ldc];
+  double o = alpha * s;
+  if (beta != 0.0 && i >= zero_rows) o += beta * (*c);
+  *c = o;
+}
+
+template <int BM, int BN>
+void launch_tile(const ld_gemm_t *g, int splits, int kchunk, double *P, hipStream_t st) {
+  dim3 grid((g->m + BM - 1) / BM, (g->n + BN - 1) / BN, splits);
+  dim3 block(NT);
+#define LGO(TA_, TB_)                                                                      \
+  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA_, TB_>), grid, block, 0, st, g->m, g->n,   \
+                     g->k, g->alpha, g->beta, g->A, g->lda, g->sa, g->B, g->ldb, g->sb,     \
+                     g->C, g->ldc, g->zero_rows, g->c_upper, kchunk, P)
+  if (!g->ta && !g->tb) LGO(false, false);
+  else if (!g->ta && g->tb) LGO(false, true);
+  else if (g->ta && !g->tb) LGO(true, false);
+  else LGO(true, true);
+#undef LGO
+}
+
+}  // namespace
+
+extern "C" void ld_gemm(const ld_gemm_t *g) {
+  if (g->m <= 0 || g->n <= 0) return;
+  hipStream_t st = (hipStream_t)ld_stream();
+  if (g->k <= 0 || g->alpha == 0.0) {
+    // C = beta C (rows < zero_rows -> 0)
+    ld_gemm_t h = *g;
+    h.k = 0;
+    launch_tile<64, 64>(&h, 1, 0, nullptr, st);
+    return;
+  }
+  const int M = g->m, N = g->n, K = g->k;
+  int bm = 128, bn = 128;
+  if (M <= 32) bm = 32; else if (M <= 64) bm = 64;
+  if (N <= 32) bn = 32; else if (N <= 64) bn = 64;
+  long tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
+  if (tiles < 256 && bm == 128 && bn == 128) {
+    bm = 64; bn = 64;
+    tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
+  }
+  int splits = 1;
+  const int target = ld_tune()->gemm_min_wgs;
+  if (tiles < target && K >= 512) {
+    splits = (int)((target + tiles - 1) / tiles);
+    int maxs = K / 256;
+    if (splits > maxs) splits = maxs;
+    // cap partial storage at 64 Mi doubles
+    long cap = (64L << 20) / ((long)M * N);
+    if (splits > cap) splits = (int)cap;
+    if (splits < 1) splits = 1;
+  }
+  int kchunk = K;
+  double *P = nullptr;
+  size_t mark = 0;
+  if (splits > 1) {
+    kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
+    splits = (K + kchunk - 1) / kchunk;
+  }
+  if (splits > 1) {
+    mark = ld_ws_mark();
+    P = (double *)ld_ws_alloc((size_t)splits * M * N * sizeof(double));
+  }
+#define DISPATCH(BM_, BN_) if (bm == BM_ && bn == BN_) launch_tile<BM_, BN_>(g, splits, kchunk, P, st)
+  DISPATCH(128, 128); else DISPATCH(128, 64); else DISPATCH(128, 32);
+  else DISPATCH(64, 128); else DISPATCH(64, 64); else DISPATCH(64, 32);
+  else DISPATCH(32, 128); else DISPATCH(32, 64); else DISPATCH(32, 32);
+#undef DISPATCH
+  if (splits > 1) {
+    size_t tot = (size_t)M * N;
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       M, N, splits, g->alpha, g->beta, P, g->C, g->ldc, g->zero_rows,
+                       g->c_upper);
+    ld_ws_release(mark);
+  }
+}

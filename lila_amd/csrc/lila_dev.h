@@ -1,0 +1,142 @@
+/*
+ * lila_dev.h -- internal C interface between the host drivers (plain C,
+ * drivers.c / api.c) and the HIP side (runtime.hip and the kernel files).
+ *
+ * Every ld_* routine enqueues work on the library stream and returns
+ * immediately; all pointers are device pointers into column-major FP64
+ * matrices.  Nothing here allocates per call except through the workspace
+ * arena (ld_ws_*), which the API layer sizes before a call tree runs.
+ */
+#ifndef LILA_DEV_H
+#define LILA_DEV_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* structure of a stored operand (applied to stored row r, column c before
+ * any transpose): general, unit lower trapezoidal (Householder V: 0 above,
+ * 1 on the diagonal), or upper triangular (T, R: 0 below the diagonal). */
+enum { LD_GEN = 0, LD_VUNIT = 1, LD_UPPER = 2, LD_LOWER = 3, LD_UUNIT = 4 };
+
+typedef struct {
+    int m, n, k;
+    double alpha, beta;
+    const double *A; int lda; int ta; int sa;
+    const double *B; int ldb; int tb; int sb;
+    double *C; int ldc;
+    int zero_rows;  /* rows [0, zero_rows) of C take beta = 0 */
+    int c_upper;    /* 1: only C(i,j) with i <= j is written (SYRK form) */
+} ld_gemm_t;
+
+/* C = alpha op(A) op(B) + beta C on FP64 MFMA (split-K chosen inside) */
+void ld_gemm(const ld_gemm_t *g);
+static inline void ld_gemm_simple(int ta, int tb, int m, int n, int k, double alpha,
+                                  const double *A, int lda, int sa,
+                                  const double *B, int ldb, int sb,
+                                  double beta, double *C, int ldc) {
+    ld_gemm_t g;
+    g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+    g.A = A; g.lda = lda; g.ta = ta; g.sa = sa;
+    g.B = B; g.ldb = ldb; g.tb = tb; g.sb = sb;
+    g.C = C; g.ldc = ldc; g.zero_rows = 0; g.c_upper = 0;
+    ld_gemm(&g);
+}
+
+/* element-wise helpers */
+void ld_lacpy(int uplo, int m, int n, const double *A, int lda, double *B, int ldb);
+void ld_laset(int uplo, int m, int n, double offdiag, double diag, double *A, int lda);
+void ld_add_diag(int n, double *A, int lda, double v);
+void ld_sub(int m, int n, const double *W, int ldw, double *C, int ldc); /* C -= W */
+
+/* Fused Householder panel for n <= LD_PANEL_MAX columns (ml >= n):
+ * V and R in A, the n x n upper T at T (ldt), and when Q != NULL the
+ * explicit panel Q = [I;0] - V T V1^T (ml x n) at Q (ldq). */
+#define LD_PANEL_MAX 32
+void ld_panel_qr(int ml, int n, double *A, int lda, double *T, int ldt,
+                 double *Q, int ldq);
+
+/* dlarft ('F','C') for k <= LD_PANEL_MAX... any k: T = fwd column T from V
+ * (ml x k, unit lower) and tau[k] (device) */
+void ld_larft(int ml, int k, const double *V, int ldv, const double *tau,
+              double *T, int ldt);
+/* copy diag of T blocks into tau (n entries, global start i) */
+void ld_T2tau(int n, int i, int mt, const double *T, int ldt, double *tau);
+/* tau_c = 2 / (1 + ||A(g+1:m, g)||^2), g = i + c */
+void ld_V2tau(int m, int n, int i, const double *A, int lda, double *tau);
+
+/* Triangular solve with op(A) (side 'L'/'R', uplo 'U'/'L', trans 'N'/'T',
+ * diag 'N'/'U'), in place on B (m x n), alpha scaling. */
+void ld_trsm(char side, char uplo, char trans, char diag, int m, int n,
+             double alpha, const double *A, int lda, double *B, int ldb);
+/* Triangular multiply B = alpha op(A) B or alpha B op(A), in place. */
+void ld_trmm(char side, char uplo, char trans, char diag, int m, int n,
+             double alpha, const double *A, int lda, double *B, int ldb);
+/* C(upper) = alpha A^T A + beta C(upper)  (A is k x n) */
+void ld_syrk_ut(int n, int k, double alpha, const double *A, int lda,
+                double beta, double *C, int ldc);
+/* upper Cholesky in place; returns through the info slot (ld_info) */
+void ld_potrf_u(int n, double *A, int lda);
+
+/* sign-chosen unpivoted LU of the n x n block Tq in place (Householder
+ * reconstruction): S[k] = -1 if |1 - t_kk| < |1 + t_kk| (column negated),
+ * then t_kk -= 1, column scaled, rank-1 update. S is a device double array. */
+void ld_lu_sign(int n, double *Tq, int ldt, double *S);
+/* negate rows r in [0, nrows) of the n columns... helpers driven by S:
+ *   ld_flip_cols: column c of X (m rows) negated when S[c] == -1
+ *   ld_flip_rows_upper: row k of X (cols [k, n)) negated when S[k] == -1
+ *   ld_flip_rows: row k of X (all n cols) negated when S[k] == -1 */
+void ld_flip_cols(int m, int n, double *X, int ldx, const double *S);
+void ld_flip_rows_upper(int n, double *X, int ldx, const double *S);
+void ld_flip_rows(int nrows, int n, double *X, int ldx, const double *S);
+/* move strict lower of Tq into A's strict lower and zero it in Tq */
+void ld_move_lower(int n, double *Tq, int ldt, double *A, int lda);
+/* dgetrf_b body (c3/lila_dgetrf_b.c): LU of [A1 D + I; A2 D] style */
+void ld_getrf_b(int ml, int n, double *A, int lda, double *Q, int ldq, double *U);
+
+/* generators (same splitmix64 stream as the oracle's) */
+void ld_fill_uniform(unsigned long long seed, int m, int n, double *A, int ld,
+                     double lo, double hi);
+void ld_fill_normal(unsigned long long seed, int m, int n, double *A, int ld);
+
+/* mixed-precision building blocks */
+void ld_gemm_lowp(int mode, const ld_gemm_t *g); /* mode 1: fp32 MFMA, 2: bf16x3 */
+
+/* ---- runtime ------------------------------------------------------------ */
+int  ld_init(void);               /* 0 on success */
+void *ld_stream(void);            /* hipStream_t */
+void ld_set_stream(void *s);
+int  ld_is_device_ptr(const void *p);
+void *ld_ws_alloc(size_t bytes);  /* bump allocation from the arena */
+size_t ld_ws_mark(void);
+void ld_ws_release(size_t mark);
+int  ld_ws_reserve(size_t bytes); /* ensure one free chunk of this size */
+void ld_ws_consolidate(void);     /* fold chunks into one (call when idle) */
+void *ld_dev_malloc(size_t bytes);
+void ld_dev_free(void *p);
+void ld_memcpy_h2d(void *dst, const void *src, size_t bytes);
+void ld_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int  ld_sync(void);
+/* status: 0 ok; >0 numerical info (e.g. potrf column); <0 errors */
+void ld_set_info(int info);
+int  ld_get_info(void);
+void ld_clear_info(void);
+/* device-side error flags (barrier timeouts, potrf breakdown), read back
+ * and folded into the info slot at ld_sync */
+int  ld_check_device_flags(void);
+void ld_fatal(const char *msg);
+
+/* tunables (autotuner writes these) */
+typedef struct {
+    int panel_rows_per_wg;   /* target rows per workgroup in the panel */
+    int gemm_min_wgs;        /* split-K target occupancy */
+    int trsm_nb;             /* diagonal block of trsm / potrf */
+} ld_tune_t;
+ld_tune_t *ld_tune(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

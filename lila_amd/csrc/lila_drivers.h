@@ -1,0 +1,50 @@
+/* lila_drivers.h -- device-pointer drivers (drivers.c) used by api.c */
+#ifndef LILA_DRIVERS_H
+#define LILA_DRIVERS_H
+
+int dv_dormqrf_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T, int ldt);
+int dv_dormqrf_z03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T, int ldt,
+                   double *B, int ldb);
+int dv_dormqrbz_w03(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *Q, int ldq,
+                    double *T, int ldt);
+int dv_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int lda, double *T, int ldt);
+int dv_dgeqr3(int m, int n, double *A, int lda, double *T, int ldt);
+int dv_dgeqr2_w03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                    int ldq);
+int dv_dgeqr2_w03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                    int ldq);
+int dv_dgeqr2_w03_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                     int ldq);
+int dv_dgeqr2_w03(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                  double *Q, int ldq);
+int dv_dorgh2_3(int m, int n, double *A, int lda, double *T, int ldt, double *Q, int ldq, double *S);
+int dv_dorgh2(int m, int n, int i, int j, int mt, double *A, int lda, double *T, int ldt, double *Q,
+              int ldq, double *S);
+int dv_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int lda, double *T, int ldt,
+                     double *Q, int ldq, double *S);
+int dv_dorghr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+              double *Sd);
+int dv_dgeqrf_w03_mt_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                        int ldq);
+int dv_dlarft_w03(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, const double *tau);
+int dv_dgeqrf_w03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                       int ldq);
+int dv_dgeqrf_w03_mt_hh(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                        int ldt, double *Q, int ldq);
+int dv_dgeqrf_w03_mt(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                     double *Q, int ldq);
+int dv_dgeqrf_w03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                            int ldt, double *Q, int ldq);
+int dv_dgeqrf_w03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
+                         int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq);
+int dv_dgetrf_b(int m, int n, int i, int mt, double *A, int lda, double *Q, int ldq, double *U);
+int dv_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb);
+
+int dq_dormqrf_w03(int m, int n, int k, int i, int mt);
+int dq_dormqrbz_w03(int m, int n, int k, int i, int mt);
+int dq_dgeqrf_w03_mt(const int *p, int m, int n, int i, int mt, int lwork);
+int dq_dgeqrf_w03_recursive(const int *p, int m, int n, int i, int mt, int lwork);
+int dq_dgeqrf_w03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
+                         int mt);
+
+#endif

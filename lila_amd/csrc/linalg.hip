@@ -1,0 +1,384 @@
+// linalg.hip -- triangular solves and products, SYRK, Cholesky, the
+// dlarft T factor and the sign-chosen LU of Householder reconstruction,
+// all on top of the FP64 MFMA GEMM (gemm_f64.hip).
+//
+// TRSM is recursive: the triangle is split at a multiple of the diagonal
+// block (64), the off-diagonal part becomes one large GEMM, and each 64 x 64
+// leaf is applied as a GEMM with its explicitly inverted diagonal block
+// (all leaves inverted up front in one launch, one workgroup per block).
+// Cholesky is recursive in the same way (TRSM + SYRK-form GEMM), with a
+// single-workgroup unblocked factorization of each 64 x 64 leaf.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include "lila_dev.h"
+
+extern "C" int *ld_device_flags(void);
+
+namespace {
+
+constexpr int TB = 64;  // diagonal block of TRSM / Cholesky leaves
+
+inline hipStream_t S() { return (hipStream_t)ld_stream(); }
+
+inline void gemm(int ta, int tb, int m, int n, int k, double alpha, const double *A, int lda,
+                 int sa, const double *B, int ldb, int sb, double beta, double *C, int ldc,
+                 int c_upper = 0) {
+  ld_gemm_t g;
+  memset(&g, 0, sizeof g);
+  g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.ta = ta; g.sa = sa;
+  g.B = B; g.ldb = ldb; g.tb = tb; g.sb = sb;
+  g.C = C; g.ldc = ldc; g.c_upper = c_upper;
+  ld_gemm(&g);
+}
+
+// inverse of each TB x TB diagonal block of the stored triangle (uplo,
+// unit) -> D[b] (TB x TB, zero outside the triangle).  One workgroup of 64
+// threads per block; thread c builds column c by substitution.
+__global__ void trtri_blocks_kernel(int nn, int upper, int unit, const double *__restrict__ A,
+                                    int lda, double *__restrict__ D) {
+  __shared__ double L[TB][TB + 1];
+  __shared__ double X[TB][TB + 1];
+  const int b = blockIdx.x, c = threadIdx.x;
+  const int o = b * TB;
+  const int bs = min(TB, nn - o);
+  for (int r = 0; r < TB; r++) {
+    double v = 0.0;
+    if (r < bs && c < bs) {
+      bool in = upper ? (r <= c) : (r >= c);
+      if (in) v = A[(size_t)(o + r) + (size_t)(o + c) * lda];
+      if (r == c && unit) v = 1.0;
+    }
+    L[r][c] = v;
+    X[r][c] = 0.0;
+  }
+  __syncthreads();
+  if (c < bs) {
+    if (!upper) {
+      X[c][c] = 1.0 / L[c][c];
+      for (int r = c + 1; r < bs; r++) {
+        double s = 0.0;
+        for (int p = c; p < r; p++) s += L[r][p] * X[p][c];
+        X[r][c] = -s / L[r][r];
+      }
+    } else {
+      X[c][c] = 1.0 / L[c][c];
+      for (int r = c - 1; r >= 0; r--) {
+        double s = 0.0;
+        for (int p = r + 1; p <= c; p++) s += L[r][p] * X[p][c];
+        X[r][c] = -s / L[r][r];
+      }
+    }
+  }
+  __syncthreads();
+  double *Db = D + (size_t)b * TB * TB;
+  for (int r = 0; r < TB; r++) Db[r + (size_t)c * TB] = X[r][c];
+}
+
+__global__ void scale_kernel(int m, int n, double alpha, double *B, int ldb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i < m && j < n) B[(size_t)i + (size_t)j * ldb] *= alpha;
+}
+
+struct TrsmCtx {
+  char side;
+  bool lowerop;  // op(A) lower triangular
+  int tA;        // op = transpose
+  const double *A;
+  int lda;
+  const double *D;  // inverted diagonal blocks
+  double *B;
+  int ldb;
+  int m, n;  // B is m x n
+};
+
+// op(A)[r0:, c0:] as a GEMM operand (pointer; transpose flag is tA)
+inline const double *opA(const TrsmCtx &t, int r0, int c0) {
+  return t.tA ? t.A + c0 + (size_t)r0 * t.lda : t.A + r0 + (size_t)c0 * t.lda;
+}
+
+// solve on the index range [o, o+len) of the triangle (multiple of TB at o)
+void trsm_rec(const TrsmCtx &t, int o, int len) {
+  if (len <= TB) {
+    const double *Db = t.D + (size_t)(o / TB) * TB * TB;
+    size_t mark = ld_ws_mark();
+    if (t.side == 'L') {
+      double *W = (double *)ld_ws_alloc(sizeof(double) * (size_t)len * t.n);
+      gemm(t.tA, 0, len, t.n, len, 1.0, Db, TB, LD_GEN, t.B + o, t.ldb, LD_GEN, 0.0, W, len);
+      ld_lacpy('A', len, t.n, W, len, t.B + o, t.ldb);
+    } else {
+      double *W = (double *)ld_ws_alloc(sizeof(double) * (size_t)t.m * len);
+      gemm(0, t.tA, t.m, len, len, 1.0, t.B + (size_t)o * t.ldb, t.ldb, LD_GEN, Db, TB, LD_GEN,
+           0.0, W, t.m);
+      ld_lacpy('A', t.m, len, W, t.m, t.B + (size_t)o * t.ldb, t.ldb);
+    }
+    ld_ws_release(mark);
+    return;
+  }
+  int nblk = (len + TB - 1) / TB;
+  int l1 = ((nblk + 1) / 2) * TB;
+  int l2 = len - l1;
+  int o1 = o, o2 = o + l1;
+  if (t.side == 'L') {
+    if (t.lowerop) {
+      trsm_rec(t, o1, l1);
+      // B2 -= op(A)[o2, o1] X1
+      gemm(t.tA, 0, l2, t.n, l1, -1.0, opA(t, o2, o1), t.lda, LD_GEN, t.B + o1, t.ldb, LD_GEN,
+           1.0, t.B + o2, t.ldb);
+      trsm_rec(t, o2, l2);
+    } else {
+      trsm_rec(t, o2, l2);
+      gemm(t.tA, 0, l1, t.n, l2, -1.0, opA(t, o1, o2), t.lda, LD_GEN, t.B + o2, t.ldb, LD_GEN,
+           1.0, t.B + o1, t.ldb);
+      trsm_rec(t, o1, l1);
+    }
+  } else {
+    if (!t.lowerop) {
+      trsm_rec(t, o1, l1);
+      // B2 -= X1 op(A)[o1, o2]
+      gemm(0, t.tA, t.m, l2, l1, -1.0, t.B + (size_t)o1 * t.ldb, t.ldb, LD_GEN, opA(t, o1, o2),
+           t.lda, LD_GEN, 1.0, t.B + (size_t)o2 * t.ldb, t.ldb);
+      trsm_rec(t, o2, l2);
+    } else {
+      trsm_rec(t, o2, l2);
+      gemm(0, t.tA, t.m, l1, l2, -1.0, t.B + (size_t)o2 * t.ldb, t.ldb, LD_GEN, opA(t, o2, o1),
+           t.lda, LD_GEN, 1.0, t.B + (size_t)o1 * t.ldb, t.ldb);
+      trsm_rec(t, o1, l1);
+    }
+  }
+}
+
+// unblocked upper Cholesky of one leaf: U = chol(A_bb(upper) - Sub), Sub
+// optional (bs x bs, ld bs); flags[1] gets the first failing global column.
+__global__ void potf2_kernel(int bs, double *A, int lda, const double *Sub, int col0, int *flags) {
+  __shared__ double U[TB][TB + 1];
+  __shared__ int bad;
+  const int c = threadIdx.x;
+  if (c == 0) bad = 0;
+  for (int r = 0; r < TB; r++) {
+    double v = 0.0;
+    if (r < bs && c < bs && r <= c) {
+      v = A[(size_t)r + (size_t)c * lda];
+      if (Sub) v -= Sub[r + (size_t)c * bs];
+    }
+    U[r][c] = v;
+  }
+  __syncthreads();
+  for (int j = 0; j < bs; j++) {
+    // U(j,j) and row j of U
+    if (c == 0) {
+      double d = U[j][j];
+      if (!(d > 0.0) && !bad) {
+        bad = 1;
+        atomicCAS(&flags[1], 0, col0 + j + 1);
+      }
+      U[j][j] = (d > 0.0) ? sqrt(d) : d;
+    }
+    __syncthreads();
+    if (c > j && c < bs) U[j][c] /= U[j][j];
+    __syncthreads();
+    if (c > j && c < bs)
+      for (int r = j + 1; r <= c; r++) U[r][c] -= U[j][r] * U[j][c];
+    __syncthreads();
+  }
+  for (int r = 0; r < bs; r++)
+    if (c < bs && r <= c) A[(size_t)r + (size_t)c * lda] = U[r][c];
+}
+
+void potrf_rec(int n, double *A, int lda, int col0, const double *Dscratch_unused) {
+  (void)Dscratch_unused;
+  if (n <= TB) {
+    hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(TB), 0, S(), n, A, lda, (const double *)nullptr,
+                       col0, ld_device_flags());
+    return;
+  }
+  int nblk = (n + TB - 1) / TB;
+  int n1 = ((nblk + 1) / 2) * TB, n2 = n - n1;
+  potrf_rec(n1, A, lda, col0, nullptr);
+  // A12 = U11^{-T} A12
+  ld_trsm('L', 'U', 'T', 'N', n1, n2, 1.0, A, lda, A + (size_t)n1 * lda, lda);
+  // A22 -= A12^T A12 (upper part only)
+  gemm(1, 0, n2, n2, n1, -1.0, A + (size_t)n1 * lda, lda, LD_GEN, A + (size_t)n1 * lda, lda, LD_GEN,
+       1.0, A + n1 + (size_t)n1 * lda, lda, 1);
+  potrf_rec(n2, A + n1 + (size_t)n1 * lda, lda, col0 + n1, nullptr);
+}
+
+// small dlarft recurrence from the Gram G = V^T V (k <= 32, one workgroup)
+__global__ void larft_small_kernel(int k, const double *__restrict__ Gm, int ldg,
+                                   const double *__restrict__ tau, double *T, int ldt) {
+  __shared__ double Tl[32][33];
+  __shared__ double tmp[32];
+  const int t = threadIdx.x;
+  for (int r = 0; r < 32; r++) Tl[r][t] = 0.0;
+  __syncthreads();
+  for (int j = 0; j < k; j++) {
+    const double tj = tau[j];
+    if (t < j) tmp[t] = -tj * Gm[t + (size_t)j * ldg];
+    __syncthreads();
+    if (t < j) {
+      double s = 0.0;
+      if (tj != 0.0)
+        for (int q = t; q < j; q++) s += Tl[t][q] * tmp[q];
+      Tl[t][j] = s;
+    }
+    if (t == j) Tl[j][j] = tj;
+    __syncthreads();
+  }
+  for (int r = 0; r < k; r++)
+    if (t < k && r <= t) T[(size_t)r + (size_t)t * ldt] = Tl[r][t];
+}
+
+// sign-chosen unpivoted LU of an n x n block (one workgroup).
+// rule 0 (dorgh2): if |1 - t_kk| < |1 + t_kk| negate column k, S = -1;
+//                  then t_kk -= 1.
+// rule 1 (dgetrf_b): if |1 - a_kk| < |1 + a_kk| keep the column, S = +1,
+//                  else negate the column, S = -1; then a_kk += 1.
+// Column negation covers rows [0, n) of the block; scaling and the rank-1
+// update cover rows [0, nrow) so a tall block factors in one pass.
+__global__ void lu_sign_kernel(int n, int nrow, double *T, int ldt, double *Sg, int rule) {
+  __shared__ double piv;
+  __shared__ int neg;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = 0; k < n; k++) {
+    if (tid == 0) {
+      double t = T[(size_t)k + (size_t)k * ldt];
+      bool c = fabs(1.0 - t) < fabs(1.0 + t);
+      if (rule == 0) { neg = c ? 1 : 0; Sg[k] = c ? -1.0 : 1.0; }
+      else { neg = c ? 0 : 1; Sg[k] = c ? 1.0 : -1.0; }
+    }
+    __syncthreads();
+    if (neg)
+      for (int r = tid; r < nrow; r += nt) T[(size_t)r + (size_t)k * ldt] = -T[(size_t)r + (size_t)k * ldt];
+    __syncthreads();
+    if (tid == 0) {
+      double t = T[(size_t)k + (size_t)k * ldt] + (rule == 0 ? -1.0 : 1.0);
+      T[(size_t)k + (size_t)k * ldt] = t;
+      piv = t;
+    }
+    __syncthreads();
+    for (int r = k + 1 + tid; r < nrow; r += nt) T[(size_t)r + (size_t)k * ldt] /= piv;
+    __syncthreads();
+    const int rows = nrow - k - 1, cols = n - k - 1;
+    for (long e = tid; e < (long)rows * cols; e += nt) {
+      int r = k + 1 + (int)(e % rows), c = k + 1 + (int)(e / rows);
+      T[(size_t)r + (size_t)c * ldt] -= T[(size_t)r + (size_t)k * ldt] * T[(size_t)k + (size_t)c * ldt];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void flip_where_kernel(int m, int n, double *X, int ldx, const double *S, double which) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i < m && j < n && S[j] == which) X[(size_t)i + (size_t)j * ldx] = -X[(size_t)i + (size_t)j * ldx];
+}
+
+__global__ void flip_urows_where_kernel(int n, double *U, int ldu, const double *S, double which) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = blockIdx.y;
+  if (k < n && c < n && c >= k && S[k] == which) U[(size_t)k + (size_t)c * ldu] = -U[(size_t)k + (size_t)c * ldu];
+}
+
+}  // namespace
+
+extern "C" {
+
+void ld_trsm(char side, char uplo, char trans, char diag, int m, int n, double alpha,
+             const double *A, int lda, double *B, int ldb) {
+  if (m <= 0 || n <= 0) return;
+  if (alpha != 1.0)
+    hipLaunchKernelGGL(scale_kernel, dim3((m + 255) / 256, n), dim3(256), 0, S(), m, n, alpha, B, ldb);
+  if (alpha == 0.0) return;
+  const int nn = (side == 'L') ? m : n;
+  const int nblk = (nn + TB - 1) / TB;
+  size_t mark = ld_ws_mark();
+  double *D = (double *)ld_ws_alloc(sizeof(double) * (size_t)nblk * TB * TB);
+  hipLaunchKernelGGL(trtri_blocks_kernel, dim3(nblk), dim3(TB), 0, S(), nn, uplo == 'U' ? 1 : 0,
+                     diag == 'U' ? 1 : 0, A, lda, D);
+  TrsmCtx t;
+  t.side = side;
+  t.tA = (trans == 'T') ? 1 : 0;
+  t.lowerop = ((uplo == 'L') != (t.tA == 1));
+  t.A = A; t.lda = lda; t.D = D; t.B = B; t.ldb = ldb; t.m = m; t.n = n;
+  trsm_rec(t, 0, nn);
+  ld_ws_release(mark);
+}
+
+void ld_trmm(char side, char uplo, char trans, char diag, int m, int n, double alpha,
+             const double *A, int lda, double *B, int ldb) {
+  if (m <= 0 || n <= 0) return;
+  int s = (uplo == 'U') ? (diag == 'U' ? LD_UUNIT : LD_UPPER) : (diag == 'U' ? LD_VUNIT : LD_LOWER);
+  int tA = (trans == 'T') ? 1 : 0;
+  size_t mark = ld_ws_mark();
+  double *W = (double *)ld_ws_alloc(sizeof(double) * (size_t)m * n);
+  if (side == 'L') gemm(tA, 0, m, n, m, alpha, A, lda, s, B, ldb, LD_GEN, 0.0, W, m);
+  else gemm(0, tA, m, n, n, alpha, B, ldb, LD_GEN, A, lda, s, 0.0, W, m);
+  ld_lacpy('A', m, n, W, m, B, ldb);
+  ld_ws_release(mark);
+}
+
+void ld_syrk_ut(int n, int k, double alpha, const double *A, int lda, double beta, double *C, int ldc) {
+  if (n <= 0) return;
+  gemm(1, 0, n, n, k, alpha, A, lda, LD_GEN, A, lda, LD_GEN, beta, C, ldc, 1);
+}
+
+void ld_potrf_u(int n, double *A, int lda) {
+  if (n <= 0) return;
+  potrf_rec(n, A, lda, 0, nullptr);
+}
+
+void ld_larft(int ml, int k, const double *V, int ldv, const double *tau, double *T, int ldt) {
+  if (k <= 0) return;
+  if (k <= 32) {
+    size_t mark = ld_ws_mark();
+    double *Gm = (double *)ld_ws_alloc(sizeof(double) * (size_t)k * k);
+    gemm(1, 0, k, k, ml, 1.0, V, ldv, LD_VUNIT, V, ldv, LD_VUNIT, 0.0, Gm, k);
+    hipLaunchKernelGGL(larft_small_kernel, dim3(1), dim3(32), 0, S(), k, Gm, k, tau, T, ldt);
+    ld_ws_release(mark);
+    return;
+  }
+  int k1 = k / 2, k2 = k - k1;
+  ld_larft(ml, k1, V, ldv, tau, T, ldt);
+  ld_larft(ml - k1, k2, V + k1 + (size_t)k1 * ldv, ldv, tau + k1, T + k1 + (size_t)k1 * ldt, ldt);
+  // T12 = -T11 (V1^T V2) T22, V1^T V2 over rows k1.. (V2 is zero above)
+  size_t mark = ld_ws_mark();
+  double *G12 = (double *)ld_ws_alloc(sizeof(double) * (size_t)k1 * k2);
+  double *H = (double *)ld_ws_alloc(sizeof(double) * (size_t)k1 * k2);
+  gemm(1, 0, k1, k2, ml - k1, 1.0, V + k1, ldv, LD_GEN, V + k1 + (size_t)k1 * ldv, ldv, LD_VUNIT, 0.0,
+       G12, k1);
+  gemm(0, 0, k1, k2, k1, 1.0, T, ldt, LD_UPPER, G12, k1, LD_GEN, 0.0, H, k1);
+  gemm(0, 0, k1, k2, k2, -1.0, H, k1, LD_GEN, T + k1 + (size_t)k1 * ldt, ldt, LD_UPPER, 0.0,
+       T + (size_t)k1 * ldt, ldt);
+  ld_ws_release(mark);
+}
+
+void ld_lu_sign(int n, double *Tq, int ldt, double *Sg) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(lu_sign_kernel, dim3(1), dim3(1024), 0, S(), n, n, Tq, ldt, Sg, 0);
+}
+
+// c3/lila_dgetrf_b.c semantics: A (ml x n) holds the orthonormal basis, Q
+// (ml x n) is flipped where the column is kept, U (n x n, ld n) holds R and
+// gets its rows flipped likewise.  Top block factored with the sign rule,
+// rows below by L2 = A2 D U'^{-1}.
+void ld_getrf_b(int ml, int n, double *A, int lda, double *Q, int ldq, double *U) {
+  if (n <= 0 || ml <= 0) return;
+  size_t mark = ld_ws_mark();
+  double *Sg = (double *)ld_ws_alloc(sizeof(double) * (size_t)n);
+  hipLaunchKernelGGL(lu_sign_kernel, dim3(1), dim3(1024), 0, S(), n, n, A, lda, Sg, 1);
+  if (ml > n) {
+    hipLaunchKernelGGL(flip_where_kernel, dim3((ml - n + 255) / 256, n), dim3(256), 0, S(), ml - n, n,
+                       A + n, lda, Sg, -1.0);
+    ld_trsm('R', 'U', 'N', 'N', ml - n, n, 1.0, A, lda, A + n, lda);
+  }
+  hipLaunchKernelGGL(flip_where_kernel, dim3((ml + 255) / 256, n), dim3(256), 0, S(), ml, n, Q, ldq,
+                     Sg, 1.0);
+  hipLaunchKernelGGL(flip_urows_where_kernel, dim3((n + 255) / 256, n), dim3(256), 0, S(), n, U, n,
+                     Sg, 1.0);
+  ld_ws_release(mark);
+}
+
+}  // extern "C"

@@ -1,0 +1,393 @@
+// runtime.hip -- library state for the LILA MI355X path: stream, device
+// workspace arena, status slot, pointer classification, and the small
+// element-wise kernels the drivers need (copies, fills, sign flips,
+// generators).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "lila_dev.h"
+
+#define HIPCHECK(x)                                                             \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      fprintf(stderr, "lila_hip: %s failed: %s (%s:%d)\n", #x,                  \
+              hipGetErrorString(e_), __FILE__, __LINE__);                       \
+      ld_set_info(-1);                                                          \
+    }                                                                           \
+  } while (0)
+
+namespace {
+// Workspace arena: a list of device chunks used as one bump stack.  A chunk
+// is never freed while a call tree runs (pointers handed out stay valid);
+// ld_ws_consolidate() folds several chunks into one after a synchronize.
+constexpr int MAXCHUNK = 32;
+struct Chunk {
+  char *p;
+  size_t cap, top;
+};
+struct State {
+  int inited = 0;
+  hipStream_t stream = nullptr;
+  Chunk ch[MAXCHUNK];
+  int nch = 0, cur = 0;
+  int info = 0;
+  int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
+  ld_tune_t tune = {384, 512, 64};
+};
+State g;
+}  // namespace
+
+extern "C" {
+
+int ld_init(void) {
+  if (g.inited) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    fprintf(stderr, "lila_hip: no HIP device visible\n");
+    return -1;
+  }
+  if (hipMalloc(&g.dflags, 64 * sizeof(int)) != hipSuccess) return -1;
+  if (hipMemset(g.dflags, 0, 64 * sizeof(int)) != hipSuccess) return -1;
+  const char *e = getenv("LILA_PANEL_ROWS");
+  if (e) g.tune.panel_rows_per_wg = atoi(e);
+  e = getenv("LILA_GEMM_MIN_WGS");
+  if (e) g.tune.gemm_min_wgs = atoi(e);
+  g.inited = 1;
+  return 0;
+}
+
+ld_tune_t *ld_tune(void) { return &g.tune; }
+
+void *ld_stream(void) { return (void *)g.stream; }
+void ld_set_stream(void *s) { g.stream = (hipStream_t)s; }
+
+int ld_is_device_ptr(const void *p) {
+  if (!p) return 0;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+static int ws_new_chunk(size_t bytes) {
+  if (g.nch >= MAXCHUNK) {
+    fprintf(stderr, "lila_hip: workspace chunk list full\n");
+    ld_set_info(-3);
+    return -1;
+  }
+  char *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    fprintf(stderr, "lila_hip: cannot allocate %zu bytes of workspace\n", bytes);
+    ld_set_info(-2);
+    return -1;
+  }
+  g.ch[g.nch].p = p;
+  g.ch[g.nch].cap = bytes;
+  g.ch[g.nch].top = 0;
+  g.nch++;
+  return 0;
+}
+
+// make sure at least `bytes` are free in one chunk above the current top
+int ld_ws_reserve(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (g.nch > 0 && g.ch[g.cur].top + bytes <= g.ch[g.cur].cap) return 0;
+  for (int c = g.cur + 1; c < g.nch; c++)
+    if (g.ch[c].cap >= bytes) return 0;
+  size_t want = bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes;
+  return ws_new_chunk(want);
+}
+
+void *ld_ws_alloc(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (g.nch == 0 && ws_new_chunk(bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes) != 0) abort();
+  if (g.ch[g.cur].top + bytes > g.ch[g.cur].cap) {
+    // move to the next chunk that fits (chunks above cur are empty)
+    int c = g.cur + 1;
+    while (c < g.nch && g.ch[c].cap < bytes) c++;
+    if (c >= g.nch) {
+      size_t want = 2 * g.ch[g.nch - 1].cap;
+      if (want < bytes) want = bytes;
+      if (ws_new_chunk(want) != 0) abort();
+      c = g.nch - 1;
+    }
+    g.cur = c;
+    g.ch[c].top = 0;
+  }
+  void *p = g.ch[g.cur].p + g.ch[g.cur].top;
+  g.ch[g.cur].top += bytes;
+  return p;
+}
+
+// mark = (chunk << 40) | top
+size_t ld_ws_mark(void) {
+  if (g.nch == 0) return 0;
+  return ((size_t)g.cur << 40) | g.ch[g.cur].top;
+}
+void ld_ws_release(size_t mark) {
+  if (g.nch == 0) return;
+  int c = (int)(mark >> 40);
+  size_t top = mark & (((size_t)1 << 40) - 1);
+  for (int k = c + 1; k < g.nch; k++) g.ch[k].top = 0;
+  g.cur = c;
+  g.ch[c].top = top;
+}
+
+// after a synchronize, with nothing in use: fold several chunks into one
+void ld_ws_consolidate(void) {
+  if (g.nch <= 1) return;
+  size_t tot = 0;
+  for (int k = 0; k < g.nch; k++) tot += g.ch[k].cap;
+  HIPCHECK(hipStreamSynchronize(g.stream));
+  for (int k = 0; k < g.nch; k++) HIPCHECK(hipFree(g.ch[k].p));
+  g.nch = 0;
+  g.cur = 0;
+  ws_new_chunk(tot);
+}
+
+void *ld_dev_malloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) {
+    ld_set_info(-2);
+    return nullptr;
+  }
+  return p;
+}
+void ld_dev_free(void *p) {
+  if (p) HIPCHECK(hipFree(p));
+}
+void ld_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  if (bytes) HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g.stream));
+}
+void ld_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  if (bytes) {
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g.stream));
+    HIPCHECK(hipStreamSynchronize(g.stream));
+  }
+}
+
+void ld_set_info(int info) {
+  if (g.info == 0 || info < 0) g.info = info;
+}
+int ld_get_info(void) { return g.info; }
+void ld_clear_info(void) { g.info = 0; }
+
+int *ld_device_flags(void) { return g.dflags; }
+
+int ld_check_device_flags(void) {
+  int h[4] = {0, 0, 0, 0};
+  if (!g.dflags) return 0;
+  HIPCHECK(hipMemcpyAsync(h, g.dflags, sizeof h, hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(hipStreamSynchronize(g.stream));
+  if (h[0]) {
+    fprintf(stderr, "lila_hip: grid barrier timed out in a panel kernel\n");
+    ld_set_info(-4);
+  }
+  if (h[1] > 0) ld_set_info(h[1]);
+  if (h[0] || h[1]) HIPCHECK(hipMemsetAsync(g.dflags, 0, sizeof h, g.stream));
+  return h[0] ? -4 : h[1];
+}
+
+int ld_sync(void) {
+  hipError_t e = hipStreamSynchronize(g.stream);
+  if (e != hipSuccess) {
+    fprintf(stderr, "lila_hip: stream error: %s\n", hipGetErrorString(e));
+    ld_set_info(-1);
+    return -1;
+  }
+  return ld_check_device_flags();
+}
+
+void ld_fatal(const char *msg) {
+  fprintf(stderr, "lila_hip: %s\n", msg);
+  ld_set_info(-5);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// element-wise kernels
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void lacpy_kernel(int uplo, int m, int n, const double *__restrict__ A, int lda,
+                             double *__restrict__ B, int ldb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  if (uplo == 'U' && i > j) return;
+  if (uplo == 'L' && i < j) return;
+  B[(size_t)i + (size_t)j * ldb] = A[(size_t)i + (size_t)j * lda];
+}
+
+__global__ void laset_kernel(int uplo, int m, int n, double off, double dg, double *A, int lda) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  if (i == j) { A[(size_t)i + (size_t)j * lda] = dg; return; }
+  if (uplo == 'U' && i > j) return;
+  if (uplo == 'L' && i < j) return;
+  A[(size_t)i + (size_t)j * lda] = off;
+}
+
+__global__ void add_diag_kernel(int n, double *A, int lda, double v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[(size_t)i + (size_t)i * lda] += v;
+}
+
+__global__ void sub_kernel(int m, int n, const double *__restrict__ W, int ldw, double *C, int ldc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i < m && j < n) C[(size_t)i + (size_t)j * ldc] -= W[(size_t)i + (size_t)j * ldw];
+}
+
+__global__ void flip_cols_kernel(int m, int n, double *X, int ldx, const double *S) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i < m && j < n && S[j] == -1.0) X[(size_t)i + (size_t)j * ldx] = -X[(size_t)i + (size_t)j * ldx];
+}
+
+__global__ void flip_rows_upper_kernel(int n, double *X, int ldx, const double *S) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;  // row
+  int c = blockIdx.y;                              // column
+  if (k < n && c < n && c >= k && S[k] == -1.0)
+    X[(size_t)k + (size_t)c * ldx] = -X[(size_t)k + (size_t)c * ldx];
+}
+
+__global__ void flip_rows_kernel(int nrows, int n, double *X, int ldx, const double *S) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = blockIdx.y;
+  if (k < nrows && c < n && S[k] == -1.0) X[(size_t)k + (size_t)c * ldx] = -X[(size_t)k + (size_t)c * ldx];
+}
+
+__global__ void move_lower_kernel(int n, double *Tq, int ldt, double *A, int lda) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = blockIdx.y;
+  if (r < n && c < n && r > c) {
+    A[(size_t)r + (size_t)c * lda] = Tq[(size_t)r + (size_t)c * ldt];
+    Tq[(size_t)r + (size_t)c * ldt] = 0.0;
+  }
+}
+
+__global__ void t2tau_kernel(int n, int i, int mt, const double *T, int ldt, double *tau) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) {
+    int g = i + c;
+    tau[c] = T[(size_t)(g % mt) + (size_t)g * ldt];
+  }
+}
+
+// one workgroup per reflector: tau = 2 / (1 + ||A(g+1:m, g)||^2)
+__global__ void v2tau_kernel(int m, int i, const double *A, int lda, double *tau) {
+  __shared__ double red[256];
+  int c = blockIdx.x, g = i + c;
+  double s = 0.0;
+  for (int r = g + 1 + threadIdx.x; r < m; r += blockDim.x) {
+    double v = A[(size_t)r + (size_t)g * lda];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tau[c] = 2.0 / (1.0 + red[0]);
+}
+
+__device__ __forceinline__ uint64_t sm64_at(uint64_t seed, uint64_t idx) {
+  // value number idx (0-based) of the splitmix64 stream started at seed
+  uint64_t z = seed + (idx + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__global__ void fill_uniform_kernel(uint64_t seed, int m, int n, double *A, int ld, double lo, double hi) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t tot = (size_t)m * n;
+  if (e >= tot) return;
+  int r = (int)(e % m), c = (int)(e / m);
+  double u = (double)(sm64_at(seed, e) >> 11) * 0x1.0p-53;
+  A[(size_t)r + (size_t)c * ld] = lo + (hi - lo) * u;
+}
+
+__global__ void fill_normal_kernel(uint64_t seed, int m, int n, double *A, int ld) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t tot = (size_t)m * n;
+  if (e >= tot) return;
+  int r = (int)(e % m), c = (int)(e / m);
+  size_t pair = e & ~(size_t)1;
+  double u1 = (double)((sm64_at(seed, pair) >> 11) + 1) * 0x1.0p-53;
+  double u2 = (double)(sm64_at(seed, pair + 1) >> 11) * 0x1.0p-53;
+  double rad = sqrt(-2.0 * log(u1));
+  double th = 6.283185307179586 * u2;
+  A[(size_t)r + (size_t)c * ld] = (e & 1) ? rad * sin(th) : rad * cos(th);
+}
+
+inline dim3 grid2(int m, int n) { return dim3((unsigned)((m + 255) / 256), (unsigned)n); }
+
+}  // namespace
+
+extern "C" {
+
+void ld_lacpy(int uplo, int m, int n, const double *A, int lda, double *B, int ldb) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(lacpy_kernel, grid2(m, n), dim3(256), 0, g.stream, uplo, m, n, A, lda, B, ldb);
+}
+void ld_laset(int uplo, int m, int n, double off, double dg, double *A, int lda) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(laset_kernel, grid2(m, n), dim3(256), 0, g.stream, uplo, m, n, off, dg, A, lda);
+}
+void ld_add_diag(int n, double *A, int lda, double v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(add_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, g.stream, n, A, lda, v);
+}
+void ld_sub(int m, int n, const double *W, int ldw, double *C, int ldc) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(sub_kernel, grid2(m, n), dim3(256), 0, g.stream, m, n, W, ldw, C, ldc);
+}
+void ld_flip_cols(int m, int n, double *X, int ldx, const double *S) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(flip_cols_kernel, grid2(m, n), dim3(256), 0, g.stream, m, n, X, ldx, S);
+}
+void ld_flip_rows_upper(int n, double *X, int ldx, const double *S) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(flip_rows_upper_kernel, grid2(n, n), dim3(256), 0, g.stream, n, X, ldx, S);
+}
+void ld_flip_rows(int nrows, int n, double *X, int ldx, const double *S) {
+  if (nrows <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(flip_rows_kernel, grid2(nrows, n), dim3(256), 0, g.stream, nrows, n, X, ldx, S);
+}
+void ld_move_lower(int n, double *Tq, int ldt, double *A, int lda) {
+  if (n <= 1) return;
+  hipLaunchKernelGGL(move_lower_kernel, grid2(n, n), dim3(256), 0, g.stream, n, Tq, ldt, A, lda);
+}
+void ld_T2tau(int n, int i, int mt, const double *T, int ldt, double *tau) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(t2tau_kernel, dim3((n + 255) / 256), dim3(256), 0, g.stream, n, i, mt, T, ldt, tau);
+}
+void ld_V2tau(int m, int n, int i, const double *A, int lda, double *tau) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(v2tau_kernel, dim3(n), dim3(256), 0, g.stream, m, i, A, lda, tau);
+}
+void ld_fill_uniform(unsigned long long seed, int m, int n, double *A, int ld, double lo, double hi) {
+  size_t tot = (size_t)m * n;
+  if (!tot) return;
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g.stream,
+                     (uint64_t)seed, m, n, A, ld, lo, hi);
+}
+void ld_fill_normal(unsigned long long seed, int m, int n, double *A, int ld) {
+  size_t tot = (size_t)m * n;
+  if (!tot) return;
+  hipLaunchKernelGGL(fill_normal_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g.stream,
+                     (uint64_t)seed, m, n, A, ld);
+}
+
+}  // extern "C"
