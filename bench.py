@@ -1,0 +1,221 @@
+"""Benchmark of the LILA Householder-QR hot path on one MI355X.
+
+Workload (one "step"): the plain FP64 drop-in -- lila_dgeqrf_w03_recursive
+(leaf = hh, panel = fused dgeqr3, VRTQ: V, R, T and the explicit Q) of one
+m = n = 16384 column-major matrix with entries iid Uniform(-1,1) from the
+splitmix64 generator, the matrix already resident in HBM (each step starts
+with a device-to-device copy of the pristine input, 2 GiB, inside the timed
+region).  Rate = the reference's VRTQ count 4mn^2 - 4/3 n^3
+(lila-c/main_w03.c:200) / time.
+
+Besides the JSON line's contract fields it reports:
+  roofline     -- the dominant kernel class (most time in the timed steps),
+                  from HIP events on the library stream around every launch:
+                  achieved = its algorithmic flops / its summed launch time,
+                  against FP64 MFMA peak (78.6 TFLOP/s);
+  cpu_baseline -- the CPU oracle (a C restatement of LILA's w03 recursion,
+                  BLAS-3 steps through the host OpenBLAS, all its threads) on
+                  a bounded sample of the same workload, rank 0 only;
+  accuracy     -- ||A - QR||_F/||A||_F and ||I - Q^T Q||_F of the last step,
+                  computed on the GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+HBM_PEAK = 8000.0      # GB/s
+METRIC = ("QR TFLOP/s mixed vs plain FP64 at backward err<=1e-13, m=n=16384; batch QR/s")
+
+
+def vrtq_flops(m, n):
+    return 4.0 * m * n * n - 4.0 / 3.0 * n ** 3
+
+
+def parse_prof(txt):
+    rows = []
+    for line in txt.strip().splitlines():
+        name, launches, flops, nbytes, ms = line.split("|")
+        rows.append(dict(name=name, launches=int(launches), flops=float(flops),
+                         bytes=float(nbytes), ms=float(ms)))
+    return rows
+
+
+def cpu_baseline(m_target, mt, nx, budget_s):
+    """oracle restatement + host OpenBLAS on a bounded square sample"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    path = pyoracle.use_host_blas()
+    nthreads = None
+    if path:
+        try:
+            ob = ctypes.CDLL(path)
+            nthreads = int(ob.scipy_openblas_get_num_threads())
+        except Exception:
+            nthreads = None
+
+    def run(nn):
+        A = pyoracle.fill_uniform(7, nn, nn, -1.0, 1.0)
+        Q = np.zeros((nn, nn), order="F")
+        T = np.zeros((mt, nn), order="F")
+        p = np.array([0, 1, 1, nx], dtype=np.int32)
+        t0 = time.perf_counter()
+        pyoracle.call("lila_dgeqrf_w03_recursive", p, nn, nn, 0, mt, A, nn, T, mt, Q, nn, None, 0)
+        return time.perf_counter() - t0
+
+    nn = 1024
+    t = run(nn)
+    rate = vrtq_flops(nn, nn) / t
+    # pick the size whose run takes about budget_s at the measured rate
+    while nn < min(8192, m_target):
+        nxt = int(nn * 1.5) // 64 * 64
+        if vrtq_flops(nxt, nxt) / rate > budget_s:
+            break
+        nn = nxt
+    t = run(nn)
+    rate = vrtq_flops(nn, nn) / t
+    pyoracle.use_builtin_blas()
+    return {
+        "value": rate / 1e12,
+        "unit": "TFLOP/s",
+        "cores": nthreads or os.cpu_count(),
+        "kind": "port",
+        "sample": ("lila_dgeqrf_w03_recursive VRTQ of one m=n=%d Uniform(-1,1) matrix, mt=%d, nx=%d, "
+                   "leaf=hh, panel=dgeqr3 (C restatement of lila-c, BLAS-3 steps on host OpenBLAS "
+                   "with %s threads), %.1f s" % (nn, mt, nx, nthreads, t)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=16384)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--mt", type=int, default=256)
+    ap.add_argument("--nx", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-prof", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the hot path runs on one MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import lila_amd
+    lib = lila_amd.load()
+    assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    dev = torch.device("cuda:0")
+    m, n, mt, nx = args.m, args.n, args.mt, args.nx
+    f64 = torch.float64
+    # column-major m x n storage: torch shape (n, m)
+    A0 = torch.empty((n, m), dtype=f64, device=dev)
+    lila_amd.call("lila_fill_uniform", ctypes.c_ulonglong(20241203), m, n, A0, m, -1.0, 1.0)
+    A = torch.empty_like(A0)
+    Q = torch.empty_like(A0)
+    T = torch.zeros((n, mt), dtype=f64, device=dev)
+    p = np.array([0, 1, 1, nx], dtype=np.int32)
+
+    def step():
+        A.copy_(A0)
+        r = lila_amd.call("lila_dgeqrf_w03_recursive", p, m, n, 0, mt, A, m, T, mt, Q, m, None, 0)
+        if r != 0:
+            raise RuntimeError("lila_dgeqrf_w03_recursive returned %d" % r)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    assert lila_amd.call("lila_get_info") == 0
+    if not args.no_prof:
+        lib.lila_prof_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    info = lila_amd.call("lila_get_info")
+    prof = []
+    if not args.no_prof:
+        buf = ctypes.create_string_buffer(1 << 16)
+        lib.lila_prof_read(buf, len(buf))
+        lib.lila_prof_enable(0)
+        prof = parse_prof(buf.value.decode())
+    dt = (t1 - t0) / args.steps
+    flops = vrtq_flops(m, n)
+    value = flops / dt / 1e12
+
+    # accuracy of the last step, on the GPU
+    R = torch.tril(A)  # upper triangle of the column-major A
+    QR = torch.empty_like(A0)
+    lila_amd.call("lila_dgemm", ctypes.c_char(b"N"), ctypes.c_char(b"N"), m, n, n, 1.0, Q, m, R, m,
+                  0.0, QR, m)
+    torch.cuda.synchronize()
+    bwd = (torch.linalg.norm(QR - A0) / torch.linalg.norm(A0)).item()
+    del QR, R
+    G = torch.eye(n, dtype=f64, device=dev)
+    lila_amd.call("lila_dgemm", ctypes.c_char(b"T"), ctypes.c_char(b"N"), n, n, m, 1.0, Q, m, Q, m,
+                  -1.0, G, n)
+    torch.cuda.synchronize()
+    orth = torch.linalg.norm(G).item()
+    del G
+
+    roof = None
+    if prof:
+        dom = max(prof, key=lambda r: r["ms"])
+        bound = "mfma" if dom["name"].startswith("gemm") else "hbm"
+        if bound == "mfma":
+            ach = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+            roof = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK, "unit": "TFLOP/s",
+                    "frac": ach / FP64_MFMA_PEAK, "traffic": None, "kernel": dom["name"],
+                    "launches": dom["launches"], "avg_launch_ms": dom["ms"] / dom["launches"],
+                    "share_of_step": dom["ms"] / (dt * 1e3 * args.steps)}
+        else:
+            ach = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s",
+                    "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom["name"],
+                    "launches": dom["launches"], "avg_launch_ms": dom["ms"] / dom["launches"],
+                    "share_of_step": dom["ms"] / (dt * 1e3 * args.steps)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(m, mt, nx, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain FP64 recursive Householder QR with explicit Q "
+                               "(lila_dgeqrf_w03_recursive, VRTQ) m=n=%d Uniform(-1,1)" % n,
+                   "m": m, "n": n, "mt": mt, "nx": nx, "leaf": "hh", "panel": "dgeqr3",
+                   "flops_per_step": flops},
+        "accuracy": {"backward_error": bwd, "orthogonality": orth, "device_status": info},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernel_classes": prof,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

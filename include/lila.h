@@ -151,6 +151,12 @@ int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, 
 int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double *A,
                int lda, const double *B, int ldb, double beta, double *C, int ldc);
 
+/* Launch profiling: when on, every launch of a hot kernel is bracketed by
+ * hipEvents on the library stream; lila_prof_read writes one line per
+ * kernel class, "name|launches|algorithmic_flops|algorithmic_bytes|ms". */
+int lila_prof_enable(int on);
+int lila_prof_read(char *buf, int buflen);
+
 /* Device generators of the test inputs (splitmix64 stream; Box-Muller). */
 int lila_fill_uniform(unsigned long long seed, int m, int n, double *A, int lda, double lo,
                       double hi);

@@ -188,6 +188,11 @@ template <int BM, int BN>
 void launch_tile(const ld_gemm_t *g, int splits, int kchunk, double *P, hipStream_t st) {
   dim3 grid((g->m + BM - 1) / BM, (g->n + BN - 1) / BN, splits);
   dim3 block(NT);
+  char cls[96];
+  snprintf(cls, sizeof cls, "gemm_f64_kernel<%d, %d, %s, %s>", BM, BN, g->ta ? "true" : "false",
+           g->tb ? "true" : "false");
+  const double M = g->m, N = g->n, K = g->k;
+  const int ph = ld_prof_begin(cls);
 #define LGO(TA_, TB_)                                                                      \
   hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA_, TB_>), grid, block, 0, st, g->m, g->n,   \
                      g->k, g->alpha, g->beta, g->A, g->lda, g->sa, g->B, g->ldb, g->sb,     \
@@ -197,6 +202,8 @@ void launch_tile(const ld_gemm_t *g, int splits, int kchunk, double *P, hipStrea
   else if (g->ta && !g->tb) LGO(true, false);
   else LGO(true, true);
 #undef LGO
+  ld_prof_end(ph, 2.0 * M * N * K,
+              8.0 * (M * K + K * N + (P ? (double)splits * M * N : 2.0 * M * N)));
 }
 
 }  // namespace
@@ -249,9 +256,11 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
 #undef DISPATCH
   if (splits > 1) {
     size_t tot = (size_t)M * N;
+    const int ph = ld_prof_begin("splitk_reduce_kernel");
     hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        M, N, splits, g->alpha, g->beta, P, g->C, g->ldc, g->zero_rows,
                        g->c_upper);
+    ld_prof_end(ph, (double)splits * tot, 8.0 * ((double)splits * tot + 2.0 * tot));
     ld_ws_release(mark);
   }
 }

@@ -128,6 +128,10 @@ void ld_clear_info(void);
 int  ld_check_device_flags(void);
 void ld_fatal(const char *msg);
 
+/* opt-in launch profiling (HIP events on the library stream) */
+int  ld_prof_begin(const char *kernel_class);  /* -1 when profiling is off */
+void ld_prof_end(int handle, double flops, double bytes);
+
 /* tunables (autotuner writes these) */
 typedef struct {
     int panel_rows_per_wg;   /* target rows per workgroup in the panel */

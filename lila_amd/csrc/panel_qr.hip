@@ -381,9 +381,13 @@ extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int l
   pa.cnt = (unsigned *)(flags + 8);
   pa.err = flags;
   void *args[] = {&pa};
+  const int ph = ld_prof_begin("panel_qr_kernel");
   hipError_t e = hipLaunchCooperativeKernel((const void *)panel_qr_kernel, dim3(G), dim3(PNT), args,
                                             (unsigned)panel_smem_bytes(nb, rp),
                                             (hipStream_t)ld_stream());
+  // algorithmic traffic: read the panel, write V/R (+ Q), write T
+  ld_prof_end(ph, 2.0 * ml * nb * nb + (Q ? 2.0 * ml * nb * nb : 0.0),
+              8.0 * ((Q ? 3.0 : 2.0) * ml * nb + (double)nb * nb));
   if (e != hipSuccess) {
     fprintf(stderr, "lila_hip: panel launch failed (G=%d rp=%d nb=%d): %s\n", G, rp, nb,
             hipGetErrorString(e));

@@ -212,6 +212,118 @@ void ld_fatal(const char *msg) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// opt-in launch profiling: a start/stop hipEvent pair on the library stream
+// around each launch of a kernel class, with the launch's algorithmic flops
+// and bytes; read back (after a synchronize) as per-class totals.
+// ---------------------------------------------------------------------------
+namespace {
+struct ProfRec {
+  int cls;
+  hipEvent_t a, b;
+  double flops, bytes;
+};
+struct ProfCls {
+  char name[96];
+  long launches;
+  double flops, bytes, ms;
+};
+int prof_on = 0;
+ProfRec *prec = nullptr;
+int nrec = 0, caprec = 0;
+ProfCls pcls[64];
+int ncls = 0;
+hipEvent_t *evpool = nullptr;
+int nev = 0, capev = 0, evused = 0;
+
+hipEvent_t ev_get() {
+  if (evused == nev) {
+    if (nev == capev) {
+      capev = capev ? 2 * capev : 4096;
+      evpool = (hipEvent_t *)realloc(evpool, sizeof(hipEvent_t) * capev);
+    }
+    hipEventCreate(&evpool[nev++]);
+  }
+  return evpool[evused++];
+}
+
+void prof_flush() {
+  for (int r = 0; r < nrec; r++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, prec[r].a, prec[r].b) == hipSuccess) {
+      ProfCls &c = pcls[prec[r].cls];
+      c.launches++;
+      c.flops += prec[r].flops;
+      c.bytes += prec[r].bytes;
+      c.ms += ms;
+    }
+  }
+  nrec = 0;
+  evused = 0;
+}
+}  // namespace
+
+extern "C" {
+
+int ld_prof_begin(const char *name) {
+  if (!prof_on) return -1;
+  int c = 0;
+  for (; c < ncls; c++)
+    if (strcmp(pcls[c].name, name) == 0) break;
+  if (c == ncls) {
+    if (ncls == 64) return -1;
+    memset(&pcls[c], 0, sizeof pcls[c]);
+    snprintf(pcls[c].name, sizeof pcls[c].name, "%s", name);
+    ncls++;
+  }
+  if (evused + 2 > 60000) {  // bound the outstanding events: drain
+    hipStreamSynchronize(g.stream);
+    prof_flush();
+  }
+  if (nrec == caprec) {
+    caprec = caprec ? 2 * caprec : 4096;
+    prec = (ProfRec *)realloc(prec, sizeof(ProfRec) * caprec);
+  }
+  ProfRec &r = prec[nrec];
+  r.cls = c;
+  r.a = ev_get();
+  r.b = ev_get();
+  r.flops = r.bytes = 0;
+  hipEventRecord(r.a, g.stream);
+  return nrec++;
+}
+
+void ld_prof_end(int h, double flops, double bytes) {
+  if (h < 0) return;
+  prec[h].flops = flops;
+  prec[h].bytes = bytes;
+  hipEventRecord(prec[h].b, g.stream);
+}
+
+// ---- exported (include/lila.h extension) ----
+int lila_prof_enable(int on) {
+  if (ld_init() != 0) return -1;
+  hipStreamSynchronize(g.stream);
+  prof_flush();
+  if (on) ncls = 0;
+  prof_on = on;
+  return 0;
+}
+
+// "name|launches|flops|bytes|ms\n" per kernel class; returns the class count
+int lila_prof_read(char *buf, int buflen) {
+  if (ld_init() != 0) return -1;
+  hipStreamSynchronize(g.stream);
+  prof_flush();
+  int off = 0;
+  for (int c = 0; c < ncls && off < buflen; c++)
+    off += snprintf(buf + off, (size_t)(buflen - off), "%s|%ld|%.6e|%.6e|%.6f\n", pcls[c].name,
+                    pcls[c].launches, pcls[c].flops, pcls[c].bytes, pcls[c].ms);
+  return ncls;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
 // element-wise kernels
 // ---------------------------------------------------------------------------
 namespace {

@@ -16,6 +16,15 @@ from conftest import U64, say, to_dev, to_host
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(scope="module", autouse=True)
+def oracle_on_host_blas(ora):
+    """the oracle's BLAS-3 steps on the host OpenBLAS for speed (its pinning
+    in test_oracle.py covers both BLAS layers)"""
+    ora.use_host_blas()
+    yield
+    ora.use_builtin_blas()
+
+
 def rel(a, b):
     d = np.linalg.norm(a - b)
     s = max(np.linalg.norm(b), 1e-300)

@@ -193,14 +193,22 @@ def test_dormqrf_z03_matches_dormqr(ora, m, k, nr, mt, ii):
                            lwork=max(1, 64 * nr))
     assert info == 0
     assert rel(Bo[ii:, ii:], cq) < 20 * k * U64
-    # planted mistake: T transposed inside each block must be caught
+    # planted mistakes in the oracle's input must be caught by this check:
+    # (a) a dropped term -- T's coupling entries (strict upper) zeroed
     Tw = T.copy(order="F")
-    for c0 in range(ii, ii + k):
-        pass
-    Tw[:, ii:] = 0.0
+    for c in range(ii, ii + k):
+        r0 = (c // mt) * mt if c // mt > ii // mt else ii
+        Tw[r0 % mt:c % mt, c] = 0.0
     Bw = B.copy(order="F")
     ora.call("lila_dormqrf_z03", m, nr, k, ii, ii, mt, Af, m, Tw, mt, Bw, m, None, 0)
-    assert rel(Bw[ii:, ii:], cq) > 1e-3
+    if min(mt, k) > 1:
+        assert rel(Bw[ii:, ii:], cq) > 1e-6
+    # (b) one wrong sign -- the first reflector's tau negated
+    Tw = T.copy(order="F")
+    Tw[ii % mt, ii] = -Tw[ii % mt, ii]
+    Bw = B.copy(order="F")
+    ora.call("lila_dormqrf_z03", m, nr, k, ii, ii, mt, Af, m, Tw, mt, Bw, m, None, 0)
+    assert rel(Bw[ii:, ii:], cq) > 1e-6
 
 
 CASES = [
@@ -424,3 +432,21 @@ def test_generators(ora):
     # distinct seeds give distinct streams, the same seed the same stream
     assert not np.array_equal(ora.fill_uniform(1, 10, 10), ora.fill_uniform(2, 10, 10))
     assert np.array_equal(ora.fill_uniform(1, 10, 10), ora.fill_uniform(1, 10, 10))
+
+
+def test_oracle_blas_layers_agree(ora):
+    """the oracle gives the same result through its built-in BLAS loops and
+    through the host OpenBLAS (used for speed by the GPU tests and the CPU
+    baseline)"""
+    case = (300, 200, 5, 32, 16, None, None)
+    out = []
+    for host in (False, True):
+        if host:
+            assert ora.use_host_blas() is not None
+        try:
+            A, Ao, To, Qo, lda, ldq = run_ora(ora, "recursive", case, 1, 1)
+        finally:
+            ora.use_builtin_blas()
+        out.append((Ao, To, Qo))
+    for x, y in zip(out[0], out[1]):
+        assert rel(x, y) < 50 * 200 * U64
