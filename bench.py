@@ -101,12 +101,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--m", type=int, default=16384)
     ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--mt", type=int, default=256)
+    ap.add_argument("--mt", type=int, default=512)
     ap.add_argument("--nx", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-prof", action="store_true")
+    ap.add_argument("--gemm-tile", default=None, help="GEMM tile override, e.g. 64x64")
     args = ap.parse_args()
+    if args.gemm_tile:
+        os.environ["LILA_GEMM_TILE"] = args.gemm_tile
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the hot path runs on one MI355X); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)

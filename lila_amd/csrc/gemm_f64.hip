@@ -45,8 +45,16 @@ __device__ __forceinline__ double mask_val(const double *__restrict__ X, int ldx
   return X[(size_t)r + (size_t)c * ldx];
 }
 
+// does a K-tile of a structured stored operand touch the triangle's edge?
+// (rows [r0, r1], cols [c0, c1] of the stored matrix, inclusive)
+__device__ __forceinline__ bool tile_needs_mask(int s, int r0, int r1, int c0, int c1) {
+  if (s == LD_GEN) return false;
+  if (s == LD_VUNIT || s == LD_LOWER) return r0 <= c1;  // some r <= c
+  return r1 >= c0;                                      // upper forms: some r >= c
+}
+
 template <int BM, int BN, bool TA, bool TB>
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(
+__global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
     int M, int N, int K, double alpha, double beta,
     const double *__restrict__ A, int lda, int sa,
     const double *__restrict__ B, int ldb, int sb,
@@ -59,8 +67,9 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(
   constexpr int WNT = BN / 32;  // along n
   constexpr int NLA = BM * BK / NT;
   constexpr int NLB = BN * BK / NT;
-  __shared__ double As[BK][BM + PA];
-  __shared__ double Bs[BK][BN + PB];
+  // double-buffered tiles, k-major: As[buf][k][m], Bs[buf][k][n]
+  __shared__ double As[2][BK][BM + PA];
+  __shared__ double Bs[2][BK][BN + PB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -69,6 +78,23 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(K, kbeg + kchunk);
+
+  // fixed per-thread load coordinates: element t of the A tile is
+  // (i, p) = (ia, pa + t*da) or (ia + t*da, pa) depending on the layout
+  // (NT is a multiple of BM, BN and BK)
+  const int a_i0 = TA ? tid / BK : tid % BM;
+  const int a_p0 = TA ? tid % BK : tid / BM;
+  const int b_j0 = TB ? tid % BN : tid / BK;
+  const int b_p0 = TB ? tid / BN : tid % BK;
+  // pointer to stored element of op(A)(m0 + a_i0, a_p0) and its t-stride
+  const double *Ab = TA ? A + a_p0 + (size_t)(m0 + a_i0) * lda : A + (m0 + a_i0) + (size_t)a_p0 * lda;
+  const size_t a_ts = TA ? (size_t)(NT / BK) * lda : (size_t)(NT / BM) * lda;
+  const size_t a_ks = TA ? 1 : (size_t)lda;  // stride of one k step
+  const double *Bb = TB ? B + (n0 + b_j0) + (size_t)b_p0 * ldb : B + b_p0 + (size_t)(n0 + b_j0) * ldb;
+  const size_t b_ts = TB ? (size_t)(NT / BN) * ldb : (size_t)(NT / BK) * ldb;
+  const size_t b_ks = TB ? (size_t)ldb : 1;
+  const bool a_rows_full = (m0 + BM <= M);
+  const bool b_cols_full = (n0 + BN <= N);
 
   d4_t acc[WMT][WNT];
 #pragma unroll
@@ -79,64 +105,83 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(
   double ra[NLA], rb[NLB];
 
   auto load_tiles = [&](int k0) {
+    const bool kfull = (k0 + BK <= kend);
+    const bool amask = TA ? tile_needs_mask(sa, k0, k0 + BK - 1, m0, m0 + BM - 1)
+                          : tile_needs_mask(sa, m0, m0 + BM - 1, k0, k0 + BK - 1);
+    const bool bmask = TB ? tile_needs_mask(sb, n0, n0 + BN - 1, k0, k0 + BK - 1)
+                          : tile_needs_mask(sb, k0, k0 + BK - 1, n0, n0 + BN - 1);
+    if (kfull && a_rows_full && !amask) {
+      const double *p = Ab + (size_t)k0 * a_ks;
 #pragma unroll
-    for (int t = 0; t < NLA; t++) {
-      int e = tid + t * NT;
-      int i, p;
-      if (!TA) { i = e % BM; p = e / BM; } else { p = e % BK; i = e / BK; }
-      int gi = m0 + i, gp = k0 + p;
-      double v = 0.0;
-      if (gi < M && gp < kend) v = TA ? mask_val(A, lda, gp, gi, sa) : mask_val(A, lda, gi, gp, sa);
-      ra[t] = v;
+      for (int t = 0; t < NLA; t++) ra[t] = p[t * a_ts];
+    } else {
+#pragma unroll
+      for (int t = 0; t < NLA; t++) {
+        const int i = TA ? a_i0 + t * (NT / BK) : a_i0;
+        const int pk = TA ? a_p0 : a_p0 + t * (NT / BM);
+        const int gi = m0 + i, gp = k0 + pk;
+        double v = 0.0;
+        if (gi < M && gp < kend) v = TA ? mask_val(A, lda, gp, gi, sa) : mask_val(A, lda, gi, gp, sa);
+        ra[t] = v;
+      }
     }
+    if (kfull && b_cols_full && !bmask) {
+      const double *p = Bb + (size_t)k0 * b_ks;
 #pragma unroll
-    for (int t = 0; t < NLB; t++) {
-      int e = tid + t * NT;
-      int j, p;
-      if (!TB) { p = e % BK; j = e / BK; } else { j = e % BN; p = e / BN; }
-      int gj = n0 + j, gp = k0 + p;
-      double v = 0.0;
-      if (gj < N && gp < kend) v = TB ? mask_val(B, ldb, gj, gp, sb) : mask_val(B, ldb, gp, gj, sb);
-      rb[t] = v;
+      for (int t = 0; t < NLB; t++) rb[t] = p[t * b_ts];
+    } else {
+#pragma unroll
+      for (int t = 0; t < NLB; t++) {
+        const int j = TB ? b_j0 : b_j0 + t * (NT / BK);
+        const int pk = TB ? b_p0 + t * (NT / BN) : b_p0;
+        const int gj = n0 + j, gp = k0 + pk;
+        double v = 0.0;
+        if (gj < N && gp < kend) v = TB ? mask_val(B, ldb, gj, gp, sb) : mask_val(B, ldb, gp, gj, sb);
+        rb[t] = v;
+      }
     }
   };
-  auto store_tiles = [&]() {
+  auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < NLA; t++) {
-      int e = tid + t * NT;
-      int i, p;
-      if (!TA) { i = e % BM; p = e / BM; } else { p = e % BK; i = e / BK; }
-      As[p][i] = ra[t];
+      const int i = TA ? a_i0 + t * (NT / BK) : a_i0;
+      const int pk = TA ? a_p0 : a_p0 + t * (NT / BM);
+      As[buf][pk][i] = ra[t];
     }
 #pragma unroll
     for (int t = 0; t < NLB; t++) {
-      int e = tid + t * NT;
-      int j, p;
-      if (!TB) { p = e % BK; j = e / BK; } else { j = e % BN; p = e / BN; }
-      Bs[p][j] = rb[t];
+      const int j = TB ? b_j0 : b_j0 + t * (NT / BK);
+      const int pk = TB ? b_p0 + t * (NT / BN) : b_p0;
+      Bs[buf][pk][j] = rb[t];
     }
   };
 
-  if (kbeg < kend) load_tiles(kbeg);
+  int buf = 0;
+  if (kbeg < kend) {
+    load_tiles(kbeg);
+    store_tiles(0);
+  }
+  __syncthreads();
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    __syncthreads();
-    store_tiles();
-    __syncthreads();
-    if (k0 + BK < kend) load_tiles(k0 + BK);
+    const bool more = (k0 + BK < kend);
+    if (more) load_tiles(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK / 4; kk++) {
       const int kr = kk * 4 + (lane >> 4);
       double af[WMT], bf[WNT];
 #pragma unroll
-      for (int a = 0; a < WMT; a++) af[a] = As[kr][wm * (BM / 2) + a * 16 + (lane & 15)];
+      for (int a = 0; a < WMT; a++) af[a] = As[buf][kr][wm * (BM / 2) + a * 16 + (lane & 15)];
 #pragma unroll
-      for (int b = 0; b < WNT; b++) bf[b] = Bs[kr][wn * (BN / 2) + b * 16 + (lane & 15)];
+      for (int b = 0; b < WNT; b++) bf[b] = Bs[buf][kr][wn * (BN / 2) + b * 16 + (lane & 15)];
 #pragma unroll
       for (int a = 0; a < WMT; a++)
 #pragma unroll
         for (int b = 0; b < WNT; b++)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
     }
+    if (more) store_tiles(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
 
   // epilogue: the f64 16x16x4 accumulator puts D row (lane/16) + 4v in lane
@@ -219,16 +264,17 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
     return;
   }
   const int M = g->m, N = g->n, K = g->k;
-  int bm = 128, bn = 128;
-  if (M <= 32) bm = 32; else if (M <= 64) bm = 64;
-  if (N <= 32) bn = 32; else if (N <= 64) bn = 64;
+  const ld_tune_t *tu = ld_tune();
+  int bm = tu->gemm_bm, bn = tu->gemm_bn;
+  if (M <= 32) bm = 32; else if (M <= 64 && bm > 64) bm = 64;
+  if (N <= 32) bn = 32; else if (N <= 64 && bn > 64) bn = 64;
   long tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
-  if (tiles < 256 && bm == 128 && bn == 128) {
+  if (tiles < tu->gemm_small_tiles && bm * bn > 64 * 64) {
     bm = 64; bn = 64;
     tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
   }
   int splits = 1;
-  const int target = ld_tune()->gemm_min_wgs;
+  const int target = tu->gemm_min_wgs;
   if (tiles < target && K >= 512) {
     splits = (int)((target + tiles - 1) / tiles);
     int maxs = K / 256;

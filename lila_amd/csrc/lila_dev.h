@@ -137,6 +137,8 @@ typedef struct {
     int panel_rows_per_wg;   /* target rows per workgroup in the panel */
     int gemm_min_wgs;        /* split-K target occupancy */
     int trsm_nb;             /* diagonal block of trsm / potrf */
+    int gemm_bm, gemm_bn;    /* default (large) GEMM tile */
+    int gemm_small_tiles;    /* below this many large tiles use 64 x 64 */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

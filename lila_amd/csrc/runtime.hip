@@ -9,7 +9,9 @@
 #include <string.h>
 #include "lila_dev.h"
 
-#define HIPCHECK(x)                                                             \
+extern "C" size_t ld_panel_scratch_doubles(void);
+
+#define HIPCHECK(x)                                                           \
   do {                                                                          \
     hipError_t e_ = (x);                                                        \
     if (e_ != hipSuccess) {                                                     \
@@ -35,7 +37,8 @@ struct State {
   int nch = 0, cur = 0;
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
-  ld_tune_t tune = {384, 512, 64};
+  double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
+  ld_tune_t tune = {384, 512, 64, 64, 128, 256};
 };
 State g;
 }  // namespace
@@ -51,10 +54,26 @@ int ld_init(void) {
   }
   if (hipMalloc(&g.dflags, 64 * sizeof(int)) != hipSuccess) return -1;
   if (hipMemset(g.dflags, 0, 64 * sizeof(int)) != hipSuccess) return -1;
+  {
+    const size_t pb = ld_panel_scratch_doubles() * sizeof(double);
+    if (hipMalloc(&g.pscratch, pb) != hipSuccess) return -1;
+    if (hipMemset(g.pscratch, 0, pb) != hipSuccess) return -1;
+  }
   const char *e = getenv("LILA_PANEL_ROWS");
   if (e) g.tune.panel_rows_per_wg = atoi(e);
   e = getenv("LILA_GEMM_MIN_WGS");
   if (e) g.tune.gemm_min_wgs = atoi(e);
+  e = getenv("LILA_GEMM_TILE");  // "BMxBN", e.g. 128x64
+  if (e) {
+    int bm = 0, bn = 0;
+    if (sscanf(e, "%dx%d", &bm, &bn) == 2 && (bm == 32 || bm == 64 || bm == 128) &&
+        (bn == 32 || bn == 64 || bn == 128)) {
+      g.tune.gemm_bm = bm;
+      g.tune.gemm_bn = bn;
+    }
+  }
+  e = getenv("LILA_GEMM_SMALL_TILES");
+  if (e) g.tune.gemm_small_tiles = atoi(e);
   g.inited = 1;
   return 0;
 }
@@ -179,6 +198,7 @@ int ld_get_info(void) { return g.info; }
 void ld_clear_info(void) { g.info = 0; }
 
 int *ld_device_flags(void) { return g.dflags; }
+double *ld_panel_scratch(void) { return g.pscratch; }
 
 int ld_check_device_flags(void) {
   int h[4] = {0, 0, 0, 0};
