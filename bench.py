@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-prof", action="store_true")
     ap.add_argument("--gemm-tile", default=None, help="GEMM tile override, e.g. 64x64")
+    ap.add_argument("--panel-mode", type=int, default=1,
+                    help="1: Cholesky-QR2 + reconstruction panel (default), 0: column Householder")
     args = ap.parse_args()
     if args.gemm_tile:
         os.environ["LILA_GEMM_TILE"] = args.gemm_tile
@@ -129,6 +131,7 @@ def main():
     Q = torch.empty_like(A0)
     T = torch.zeros((n, mt), dtype=f64, device=dev)
     p = np.array([0, 1, 1, nx], dtype=np.int32)
+    lib.lila_set_panel_mode(args.panel_mode)
 
     def step():
         A.copy_(A0)
@@ -210,7 +213,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain FP64 recursive Householder QR with explicit Q "
                                "(lila_dgeqrf_w03_recursive, VRTQ) m=n=%d Uniform(-1,1)" % n,
-                   "m": m, "n": n, "mt": mt, "nx": nx, "leaf": "hh", "panel": "dgeqr3",
+                   "m": m, "n": n, "mt": mt, "nx": nx, "leaf": "hh",
+                   "panel": ("dgeqr3 via Cholesky-QR2 + Householder reconstruction"
+                             if args.panel_mode else "dgeqr3 column by column"),
                    "flops_per_step": flops},
         "accuracy": {"backward_error": bwd, "orthogonality": orth, "device_status": info},
         "roofline": roof,

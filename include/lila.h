@@ -151,6 +151,12 @@ int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, 
 int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double *A,
                int lda, const double *B, int ldb, double beta, double *C, int ldc);
 
+/* Panel algorithm of the drivers (returns the previous one): 1 (default)
+ * Cholesky-QR2 + Householder reconstruction in one fused launch, with the
+ * column-by-column kernel as in-kernel fallback; 0 always column-by-column
+ * (dgeqr3's arithmetic).  Both give dgeqr3's V, R, T and Q to rounding. */
+int lila_set_panel_mode(int mode);
+
 /* Launch profiling: when on, every launch of a hot kernel is bracketed by
  * hipEvents on the library stream; lila_prof_read writes one line per
  * kernel class, "name|launches|algorithmic_flops|algorithmic_bytes|ms". */

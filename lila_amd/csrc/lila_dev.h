@@ -57,6 +57,9 @@ void ld_sub(int m, int n, const double *W, int ldw, double *C, int ldc); /* C -=
 #define LD_PANEL_MAX 32
 void ld_panel_qr(int ml, int n, double *A, int lda, double *T, int ldt,
                  double *Q, int ldq);
+/* the same, always column-by-column (dgeqr3's own arithmetic) */
+void ld_panel_qr_exact(int ml, int n, double *A, int lda, double *T, int ldt,
+                       double *Q, int ldq);
 
 /* dlarft ('F','C') for k <= LD_PANEL_MAX... any k: T = fwd column T from V
  * (ml x k, unit lower) and tau[k] (device) */
@@ -134,11 +137,13 @@ void ld_prof_end(int handle, double flops, double bytes);
 
 /* tunables (autotuner writes these) */
 typedef struct {
-    int panel_rows_per_wg;   /* target rows per workgroup in the panel */
+    int panel_rows_per_wg;   /* target rows per workgroup, column-by-column panel */
+    int panel_rows_cqr;      /* target rows per workgroup, Cholesky-QR2 panel */
     int gemm_min_wgs;        /* split-K target occupancy */
     int trsm_nb;             /* diagonal block of trsm / potrf */
     int gemm_bm, gemm_bn;    /* default (large) GEMM tile */
     int gemm_small_tiles;    /* below this many large tiles use 64 x 64 */
+    int panel_mode;          /* 0 column Householder, 1 Cholesky-QR2 + reconstruction */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

@@ -1,30 +1,41 @@
-// panel_qr.hip -- fused Householder panel for the LILA hot path (gfx950).
+// panel_qr.hip -- fused Householder panels for the LILA hot path (gfx950).
 //
-// One launch factors an ml x nb panel (nb <= 32): the row range is split
-// into G slabs, one workgroup each, and every slab lives in LDS for the
-// whole factorization.  Column j costs ONE grid-wide reduction: each slab
-// contributes, in a single pass over its rows,
-//   * sum and max of x^2 for x = A(j+1:, j)  (the norm; the max guards the
-//     plain sum of squares against over/underflow, with a scaled second
-//     pass when it is outside [1e-150, 1e150]),
-//   * the dot products x^T A(j+1:, k) for k > j,
-//   * the dot products v_p^T v_{j-1} (p < j-1) of the previous reflector
-//     (the Gram entries the T factor needs),
-// accumulated with hardware FP64 atomic adds into a triple-buffered vector,
-// and after the barrier every workgroup forms beta, tau and
-//   v^T a_k = a_jk + (x^T a_k) / (alpha - beta)
-// itself (identical arithmetic in every workgroup), so the reflector is
-// generated (LAPACK dlarfg convention: beta = -sign(alpha) ||[alpha;x]||,
-// tau = (beta - alpha)/beta, v = x/(alpha - beta)) and applied without a
-// second barrier.  T = dlarft('F','C') is built from the Gram entries, and
-// the explicit panel Q = [I;0] - V (T V1^T) is written in the same launch.
+// An ml x nb panel (nb <= 32) is split into G row slabs, one workgroup
+// each; every slab lives in LDS for the whole launch.  Two kernels:
 //
-// Cross-workgroup data only moves through agent-scope atomics (adds, max,
-// coherent loads/stores); a barrier is one relaxed add and a poll by one
-// thread per workgroup after an s_waitcnt, with a bounded wait that raises
-// a device flag instead of hanging.  The atomic sums make the reduction order (and
-// so the last bits of the result) vary from run to run.
+// 1. panel_hh_kernel -- column-by-column Householder (dgeqr3's numbers):
+//    column j costs ONE grid-wide reduction: each slab contributes, in one
+//    pass over its rows, sum and max of x^2 for x = A(j+1:, j) (the max
+//    guards the plain sum of squares, with a scaled second pass outside
+//    [1e-150, 1e150]), the dot products x^T A(j+1:, k) for k > j, and the
+//    Gram entries v_p^T v_{j-1} the T factor needs; every workgroup then
+//    forms beta, tau (LAPACK dlarfg convention: beta = -sign(alpha)
+//    ||[alpha;x]||, tau = (beta-alpha)/beta, v = x/(alpha-beta)) and
+//    v^T a_k = a_jk + (x^T a_k)/(alpha - beta) itself and applies the
+//    reflector.  T = dlarft('F','C') comes from the Gram entries and the
+//    explicit panel Q = [I;0] - V (T V1^T) is written in the same launch.
+//    With G = 1 the reductions stay in LDS (no global traffic).
+//
+// 2. panel_cqr_kernel -- communication-avoiding: Cholesky-QR twice
+//    (Gram X^T X of each slab on v_mfma_f64_16x16x4, one grid reduction per
+//    pass, the 32 x 32 Cholesky in registers of one wave, Q = X R^-1 by
+//    row-wise substitution with the rows in registers), then Householder
+//    reconstruction: sign-chosen LU of the top nb x nb block of Q
+//    (Q1 S - I = L U), V1 = L, V2 = Q2 S U^-1, T = -U V1^-T, R = S R2 R1,
+//    explicit Q = Q S.  Three grid reductions per panel instead of nb + 1.  In exact arithmetic this is the same V, R, T
+//    and Q as the column-by-column Householder panel (Q and R are unique up
+//    to column signs and the reconstruction picks dlarfg's signs).  If a
+//    Cholesky pivot is not safely positive, the diagonal of R spreads by
+//    more than 1e5, or a reflector has tau within 1e-10 of 0 or 2, every
+//    workgroup reloads its slab and runs the column-by-column loop instead.
+//
+// Cross-workgroup data only moves through agent-scope atomics (FP64 adds,
+// max, coherent loads/stores); a barrier is an s_waitcnt, one relaxed add
+// and a poll by one thread per workgroup, with a bounded wait that raises
+// a device flag instead of hanging.  The atomic sums make the reduction
+// order (and so the last bits of the result) vary from run to run.
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -33,21 +44,37 @@
 extern "C" int *ld_device_flags(void);
 extern "C" double *ld_panel_scratch(void);
 
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
 namespace {
 
 constexpr int PNT = 256;
 constexpr int PNB = LD_PANEL_MAX;  // 32
 constexpr int SLOT = 128;          // one reduction vector
 constexpr int SLOT_SCALED = 126;   // slot of the scaled sum of squares
-constexpr int RMAX = 480;          // rows per slab held in LDS
+constexpr int LDM = 34;            // leading dimension of 32 x 32 LDS matrices (even: 16-byte pairs)
+// persistent global scratch layout (doubles)
+constexpr int OFF_RED = 0;                   // 3 x SLOT reduction vectors
+constexpr int OFF_V1 = OFF_RED + 3 * SLOT;   // V1 staging (PNB*PNB)
+constexpr int OFF_G1 = OFF_V1 + PNB * PNB;   // Gram, pass 1 (PNB*PNB)
+constexpr int OFF_G2 = OFF_G1 + PNB * PNB;   // Gram, pass 2
+constexpr int OFF_PUB = OFF_G2 + PNB * PNB;  // U^-1 (PNB*PNB), S (PNB), flag
+constexpr int SCRATCH = OFF_PUB + PNB * PNB + PNB + 8;
+// LDS past the slab (doubles): the larger of the two kernels' needs
+constexpr int HH_TAIL = 3 * PNB * PNB + 64 + 128 + 3 * PNB + 16 + SLOT;
+constexpr int CQR_TAIL = 3 * PNB * LDM + 3 * PNB + 2;
+constexpr int LDS_TAIL = (HH_TAIL > CQR_TAIL ? HH_TAIL : CQR_TAIL) + 2;
+// dynamic LDS budget: 160 KiB minus the kernels' static LDS
+constexpr int LDS_DYN_BYTES = 163840 - 16;
+constexpr int LDS_SLAB_DOUBLES = LDS_DYN_BYTES / 8 - LDS_TAIL;
 
 struct PanelArgs {
   int ml, nb;
   double *A; int lda;
   double *T; int ldt;
   double *Q; int ldq;
-  int G, rp;
-  double *red;        // 3 * SLOT reduction vectors + PNB*PNB V1 staging
+  int G, rp, rs;      // workgroups, rows per slab, LDS column stride
+  double *scr;        // persistent global scratch (SCRATCH doubles)
   unsigned *cnt;      // [0] arrivals, [1] exits
   int *err;
 };
@@ -94,6 +121,12 @@ __device__ void grid_sync(unsigned *cnt, unsigned target, int *err) {
   __syncthreads();
 }
 
+// LDS ordering between the lanes of one wave (wave-only 32 x 32 routines)
+__device__ __forceinline__ void wfence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -105,11 +138,38 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(PNT) void panel_qr_kernel(PanelArgs a) {
-  extern __shared__ double smem[];
-  const int nb = a.nb, RP = a.rp, G = a.G;
-  double *slab = smem;              // nb * RP, column-major
-  double *Gm = slab + nb * RP;      // Gram, Gm[p + PNB*j]
+// slab size in doubles, rounded up to keep what follows 16-byte aligned
+__device__ __forceinline__ size_t slab_doubles(int nb, int rs) { return ((size_t)nb * rs + 1) & ~(size_t)1; }
+
+// rows [r0, r0 + nloc) of an nb-column block of a column-major matrix into
+// the slab and back: one row per thread, eight columns in flight
+__device__ void load_slab(const PanelArgs &a, double *slab, int r0, int nloc) {
+  for (int lr = threadIdx.x; lr < nloc; lr += PNT) {
+    const double *src = a.A + (size_t)(r0 + lr);
+#pragma unroll 8
+    for (int c = 0; c < a.nb; c++) slab[lr + c * a.rs] = src[(size_t)c * a.lda];
+  }
+  __syncthreads();
+}
+__device__ void store_slab(const double *slab, int rs, int nb, int nloc, double *dst, int ld,
+                           const double *colscale) {
+  for (int lr = threadIdx.x; lr < nloc; lr += PNT) {
+    double *d = dst + lr;
+#pragma unroll 8
+    for (int c = 0; c < nb; c++)
+      d[(size_t)c * ld] = colscale ? slab[lr + c * rs] * colscale[c] : slab[lr + c * rs];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column-by-column Householder on the slab in LDS (+ T, Q, write-back).
+// SOLO: G == 1, reductions stay in LDS.
+// ---------------------------------------------------------------------------
+template <bool SOLO>
+__device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0, int nloc,
+                          unsigned &nbar) {
+  const int nb = a.nb, RS = a.rs, G = a.G;
+  double *Gm = tail;                // Gram, Gm[p + PNB*j]
   double *Tl = Gm + PNB * PNB;      // T
   double *Xl = Tl + PNB * PNB;      // V1, then T V1^T
   double *tmp = Xl + PNB * PNB;     // 64 scratch
@@ -118,26 +178,17 @@ __global__ __launch_bounds__(PNT) void panel_qr_kernel(PanelArgs a) {
   double *tauL = rowj + PNB;        // tau
   double *wk = tauL + PNB;          // w_k
   double *scal = wk + PNB;          // [0] beta [1] tau [2] scale [3] xnorm [4] maxabs
-
+  double *lred = scal + 16;         // SOLO: local reduction vector (SLOT)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = blockIdx.x;
-  const int r0 = g * RP;
-  const int r1 = min(a.ml, r0 + RP);
-  const int nloc = max(0, r1 - r0);
+  double *gred = a.scr + OFF_RED;
 
-  for (int idx = tid; idx < nloc * nb; idx += PNT) {
-    int lr = idx % nloc, c = idx / nloc;
-    slab[lr + c * RP] = a.A[(size_t)(r0 + lr) + (size_t)c * a.lda];
-  }
-  __syncthreads();
-
-  unsigned nbar = 0;
   for (int j = 0; j <= nb; j++) {
     const int nk = (j < nb) ? nb - 1 - j : 0;
     const int nr = (j < nb) ? nb - j : 0;
     const int ng = (j >= 1) ? j - 1 : 0;
-    double *rv = a.red + (size_t)(j % 3) * SLOT;
-    if (g == 0 && tid < SLOT) st_coh(&a.red[(size_t)((j + 1) % 3) * SLOT + tid], 0.0);
+    double *rv = SOLO ? lred : gred + (size_t)(j % 3) * SLOT;
+    if (!SOLO && g == 0 && tid < SLOT) st_coh(&gred[(size_t)((j + 1) % 3) * SLOT + tid], 0.0);
 
     // ---- local sums: job 0 = norm (j < nb), then nk dots, then ng Gram
     const int njob = (j < nb ? 1 : 0) + nk + ng;
@@ -146,83 +197,109 @@ __global__ __launch_bounds__(PNT) void panel_qr_kernel(PanelArgs a) {
         double s = 0.0, mx = 0.0;
         for (int lr = lane; lr < nloc; lr += 64) {
           if (r0 + lr <= j) continue;
-          double x = slab[lr + j * RP];
+          const double x = slab[lr + j * RS];
           s += x * x;
           mx = fmax(mx, fabs(x));
         }
         s = wave_sum(s);
         mx = wave_max(mx);
         if (lane == 0) {
-          if (s != 0.0) add_coh(&rv[0], s);
-          if (mx != 0.0) max_coh(&rv[1], mx);
+          if (SOLO) {
+            rv[0] = s;
+            rv[1] = mx;
+          } else {
+            if (s != 0.0) add_coh(&rv[0], s);
+            if (mx != 0.0) max_coh(&rv[1], mx);
+          }
         }
         continue;
       }
       const int qq = q - (j < nb ? 1 : 0);
       double s = 0.0;
+      int slot;
       if (qq < nk) {
         const int k = j + 1 + qq;
         for (int lr = lane; lr < nloc; lr += 64)
-          if (r0 + lr > j) s += slab[lr + j * RP] * slab[lr + k * RP];
-        s = wave_sum(s);
-        if (lane == 0 && s != 0.0) add_coh(&rv[2 + qq], s);
+          if (r0 + lr > j) s += slab[lr + j * RS] * slab[lr + k * RS];
+        slot = 2 + qq;
       } else {
         const int p = qq - nk, c = j - 1;
         for (int lr = lane; lr < nloc; lr += 64) {
           const int r = r0 + lr;
-          if (r > c) s += slab[lr + p * RP] * slab[lr + c * RP];
-          else if (r == c) s += slab[lr + p * RP];
+          if (r > c) s += slab[lr + p * RS] * slab[lr + c * RS];
+          else if (r == c) s += slab[lr + p * RS];
         }
-        s = wave_sum(s);
-        if (lane == 0 && s != 0.0) add_coh(&rv[2 + nk + nr + p], s);
+        slot = 2 + nk + nr + p;
+      }
+      s = wave_sum(s);
+      if (lane == 0) {
+        if (SOLO) rv[slot] = s;
+        else if (s != 0.0) add_coh(&rv[slot], s);
       }
     }
     if (g == 0) {
-      for (int t = tid; t < nr; t += PNT) st_coh(&rv[2 + nk + t], slab[j + (j + t) * RP]);
-    }
-    if (j == nb && g == 0) {
-      // stage V1 (strict lower of the top nb x nb) for every workgroup
-      double *v1 = a.red + 3 * SLOT;
-      for (int t = tid; t < nb * nb; t += PNT) {
-        const int r = t % nb, c = t / nb;
-        st_coh(&v1[t], r > c ? slab[r + c * RP] : 0.0);
+      for (int t = tid; t < nr; t += PNT) {
+        if (SOLO) rv[2 + nk + t] = slab[j + (j + t) * RS];
+        else st_coh(&rv[2 + nk + t], slab[j + (j + t) * RS]);
       }
     }
-    nbar++;
-    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    if (j == nb && g == 0 && !SOLO) {
+      // stage V1 (strict lower of the top nb x nb) for every workgroup
+      double *v1 = a.scr + OFF_V1;
+      for (int t = tid; t < nb * nb; t += PNT) {
+        const int r = t % nb, c = t / nb;
+        st_coh(&v1[t], r > c ? slab[r + c * RS] : 0.0);
+      }
+    }
+    if (SOLO) {
+      __syncthreads();
+    } else {
+      nbar++;
+      grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    }
 
     // ---- read the reduced vector ----
     const int nsum = nk + ng;
     if (tid < nsum) {
       const int slot = (tid < nk) ? 2 + tid : 2 + nk + nr + (tid - nk);
-      const double v = ld_coh(&rv[slot]);
+      const double v = SOLO ? rv[slot] : ld_coh(&rv[slot]);
       if (tid < nk) vals[tid] = v;
       else vals[64 + (tid - nk)] = v;
     }
-    if (tid >= 64 && tid < 64 + nr) rowj[tid - 64] = ld_coh(&rv[2 + nk + (tid - 64)]);
+    if (tid >= 64 && tid < 64 + nr)
+      rowj[tid - 64] = SOLO ? rv[2 + nk + (tid - 64)] : ld_coh(&rv[2 + nk + (tid - 64)]);
     if (tid == 128 && j < nb) {
-      scal[3] = ld_coh(&rv[0]);
-      scal[4] = ld_coh(&rv[1]);
+      scal[3] = SOLO ? rv[0] : ld_coh(&rv[0]);
+      scal[4] = SOLO ? rv[1] : ld_coh(&rv[1]);
     }
     __syncthreads();
     if (j < nb) {
       const double mx = scal[4];
       if (mx != 0.0 && (mx < 1e-150 || mx > 1e150)) {
         // scaled second pass: sum of (x / max)^2
+        if (SOLO && tid == 0) rv[SLOT_SCALED] = 0.0;
+        __syncthreads();
         if (wave == 0) {
           const double inv = 1.0 / mx;
           double s = 0.0;
           for (int lr = lane; lr < nloc; lr += 64) {
             if (r0 + lr <= j) continue;
-            const double x = slab[lr + j * RP] * inv;
+            const double x = slab[lr + j * RS] * inv;
             s += x * x;
           }
           s = wave_sum(s);
-          if (lane == 0 && s != 0.0) add_coh(&rv[SLOT_SCALED], s);
+          if (lane == 0) {
+            if (SOLO) rv[SLOT_SCALED] = s;
+            else if (s != 0.0) add_coh(&rv[SLOT_SCALED], s);
+          }
         }
-        nbar++;
-        grid_sync(a.cnt, nbar * (unsigned)G, a.err);
-        if (tid == 0) scal[3] = mx * sqrt(ld_coh(&rv[SLOT_SCALED]));
+        if (SOLO) {
+          __syncthreads();
+        } else {
+          nbar++;
+          grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+        }
+        if (tid == 0) scal[3] = mx * sqrt(SOLO ? rv[SLOT_SCALED] : ld_coh(&rv[SLOT_SCALED]));
       } else if (tid == 0) {
         scal[3] = sqrt(scal[3]);
       }
@@ -239,7 +316,9 @@ __global__ __launch_bounds__(PNT) void panel_qr_kernel(PanelArgs a) {
         tau = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
       }
-      scal[0] = beta; scal[1] = tau; scal[2] = scale;
+      scal[0] = beta;
+      scal[1] = tau;
+      scal[2] = scale;
       tauL[j] = tau;
     }
     __syncthreads();
@@ -256,12 +335,12 @@ __global__ __launch_bounds__(PNT) void panel_qr_kernel(PanelArgs a) {
       for (int lr = tid; lr < nloc; lr += PNT) {
         const int r = r0 + lr;
         if (r > j) {
-          const double v = slab[lr + j * RP] * scale;
-          slab[lr + j * RP] = v;
-          for (int k = j + 1; k < nb; k++) slab[lr + k * RP] -= v * wk[k];
+          const double v = slab[lr + j * RS] * scale;
+          slab[lr + j * RS] = v;
+          for (int k = j + 1; k < nb; k++) slab[lr + k * RS] -= v * wk[k];
         } else if (r == j) {
-          slab[lr + j * RP] = scal[0];
-          for (int k = j + 1; k < nb; k++) slab[lr + k * RP] -= wk[k];
+          slab[lr + j * RS] = scal[0];
+          for (int k = j + 1; k < nb; k++) slab[lr + k * RS] -= wk[k];
         }
       }
     }
@@ -269,10 +348,7 @@ __global__ __launch_bounds__(PNT) void panel_qr_kernel(PanelArgs a) {
   }
 
   // ---- write V and R back ----
-  for (int idx = tid; idx < nloc * nb; idx += PNT) {
-    const int lr = idx % nloc, c = idx / nloc;
-    a.A[(size_t)(r0 + lr) + (size_t)c * a.lda] = slab[lr + c * RP];
-  }
+  store_slab(slab, RS, nb, nloc, a.A + r0, a.lda, nullptr);
 
   // ---- T = dlarft('F','C') from tau and the Gram entries ----
   for (int t = tid; t < PNB * PNB; t += PNT) Tl[t] = 0.0;
@@ -299,10 +375,10 @@ __global__ __launch_bounds__(PNT) void panel_qr_kernel(PanelArgs a) {
 
   // ---- explicit panel Q = [I;0] - V (T V1^T) ----
   if (a.Q) {
-    const double *v1 = a.red + 3 * SLOT;
+    const double *v1 = a.scr + OFF_V1;
     for (int t = tid; t < nb * nb; t += PNT) {
       const int r = t % nb, c = t / nb;
-      Xl[r + PNB * c] = ld_coh(&v1[t]);  // V1 strict lower
+      Xl[r + PNB * c] = SOLO ? (r > c ? slab[r + c * RS] : 0.0) : ld_coh(&v1[t]);
     }
     __syncthreads();
     // X(r,c) = sum_{p=r..c} T(r,p) V1(c,p), V1(c,c) = 1; stored over V1's
@@ -331,60 +407,468 @@ __global__ __launch_bounds__(PNT) void panel_qr_kernel(PanelArgs a) {
       double s = (r == c) ? 1.0 : 0.0;
       const int pmax = min(r, c);
       for (int p = 0; p <= pmax; p++) {
-        const double v = (r == p) ? 1.0 : slab[lr + p * RP];
+        const double v = (r == p) ? 1.0 : slab[lr + p * RS];
         s -= v * Xl[p + PNB * c];
       }
       a.Q[(size_t)r + (size_t)c * a.ldq] = s;
     }
   }
+}
 
-  // ---- last workgroup out re-zeroes the reduction vectors and counters ----
+// last workgroup out re-zeroes the accumulation buffers and the counters
+__device__ void exit_protocol(const PanelArgs &a) {
   __syncthreads();
   __shared__ unsigned last;
+  const int tid = threadIdx.x;
   if (tid == 0) {
-    const unsigned old = __hip_atomic_fetch_add(&a.cnt[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (old == (unsigned)G - 1) ? 1u : 0u;
+    const unsigned old = __hip_atomic_fetch_add(&a.cnt[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    last = (old == (unsigned)a.G - 1) ? 1u : 0u;
   }
   __syncthreads();
   if (last) {
-    for (int t = tid; t < 3 * SLOT; t += PNT) st_coh(&a.red[t], 0.0);
+    for (int t = tid; t < 3 * SLOT; t += PNT) st_coh(&a.scr[OFF_RED + t], 0.0);
+    for (int t = tid; t < 2 * PNB * PNB; t += PNT) st_coh(&a.scr[OFF_G1 + t], 0.0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
       __hip_atomic_store(&a.cnt[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&a.cnt[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&a.cnt[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
 
-size_t panel_smem_bytes(int nb, int rp) {
-  return sizeof(double) * ((size_t)nb * rp + 3 * PNB * PNB + 64 + 128 + 3 * PNB + 8);
+template <bool SOLO>
+__global__ __launch_bounds__(PNT) void panel_hh_kernel(PanelArgs a) {
+  extern __shared__ double smem[];
+  const int g = blockIdx.x;
+  const int r0 = g * a.rp;
+  const int nloc = max(0, min(a.ml, r0 + a.rp) - r0);
+  double *slab = smem, *tail = smem + slab_doubles(a.nb, a.rs);
+  load_slab(a, slab, r0, nloc);
+  unsigned nbar = 0;
+  hh_factor<SOLO>(a, slab, tail, r0, nloc, nbar);
+  if (!SOLO) exit_protocol(a);
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky-QR2 + Householder reconstruction
+// ---------------------------------------------------------------------------
+
+// Gram X^T X of the slab (nloc x nb, stride RS) into M (ld LDM) on MFMA:
+// wave w owns the 16x16 output tile (w & 1, w >> 1); 4 independent
+// accumulators over the row dimension.  (A operand lane i + 16 kk carries
+// X[k0 + kk][ti*16 + i], B operand lane j + 16 kk carries X[k0 + kk][tj*16 + j];
+// result lane l, register v holds D[(l/16) + 4v][l%16].)
+__device__ void slab_gram(const double *X, int RS, int nloc, int nb, double *M) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ti = wave & 1, tj = wave >> 1;
+  const int ci = ti * 16 + (lane & 15), cj = tj * 16 + (lane & 15);
+  d4_t acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) acc[u] = d4_t{0.0, 0.0, 0.0, 0.0};
+  const bool vi = ci < nb, vj = cj < nb;
+  int k0 = 0;
+  for (; k0 + 16 <= nloc; k0 += 16) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int k = k0 + 4 * u + (lane >> 4);
+      const double x = vi ? X[k + ci * RS] : 0.0;
+      const double y = vj ? X[k + cj * RS] : 0.0;
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[u], 0, 0, 0);
+    }
+  }
+  for (; k0 < nloc; k0 += 4) {
+    const int k = k0 + (lane >> 4);
+    const double x = (vi && k < nloc) ? X[k + ci * RS] : 0.0;
+    const double y = (vj && k < nloc) ? X[k + cj * RS] : 0.0;
+    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[0], 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const double d = (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
+    const int r = ti * 16 + (lane >> 4) + 4 * v, c = tj * 16 + (lane & 15);
+    M[r * LDM + c] = d;
+  }
+}
+
+// v from lane l of the wave (l a compile-time constant after unrolling:
+// two v_readlane_b32, the result lives in SGPRs)
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// one wave: upper Cholesky of M (32 x 32, ld LDM, upper part read; rows
+// and columns >= nb hold the identity) in place, lower part zeroed, and the
+// reciprocals of R's diagonal into D.  Lane c keeps column c in registers;
+// step j scales row j by 1/R(j,j) (LAPACK dpotf2's scaling) and updates
+// A(r,c) -= R(j,r) R(j,c) with R(j,r) read from lane r.  Returns false if a
+// pivot is not positive or R's first nb diagonal entries spread by more
+// than 1e5.
+__device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb) {
+  const int lane = threadIdx.x & 63, c = lane & 31;
+  double a[PNB];
+#pragma unroll
+  for (int r = 0; r < PNB; r++) a[r] = M[r * LDM + c];
+  bool ok = true;
+  double dmax = 0.0, dmin = 1e308, myinv = 1.0;
+#pragma unroll
+  for (int j = 0; j < PNB; j++) {
+    const double d = rdlane(a[j], j);
+    ok = ok && (d > 0.0);
+    // 1 / sqrt(d) from the hardware estimate and two Newton steps
+    // y <- y (3 - d y^2) / 2 (quadratic convergence: full precision), then
+    // sqrt(d) = d / sqrt(d); both within an ulp or two of the rounded values
+    const double dd = (d > 0.0) ? d : 1.0;
+    double pinv = __builtin_amdgcn_rsq(dd);
+    pinv = pinv * (1.5 - 0.5 * dd * pinv * pinv);
+    pinv = pinv * (1.5 - 0.5 * dd * pinv * pinv);
+    const double p = dd * pinv;
+    if (j < nb) {
+      dmax = fmax(dmax, p);
+      dmin = fmin(dmin, p);
+    }
+    if (c == j) myinv = pinv;
+    const double rj = (c == j) ? p : a[j] * pinv;
+    a[j] = rj;
+#pragma unroll
+    for (int r = j + 1; r < PNB; r++) a[r] -= rdlane(rj, r) * rj;
+  }
+  if (lane < PNB) {
+#pragma unroll
+    for (int r = 0; r < PNB; r++) M[r * LDM + c] = (r <= c) ? a[r] : 0.0;
+    D[c] = myinv;
+  }
+  wfence();
+  return ok && dmin > 1e-5 * dmax;
+}
+
+// X (nloc rows of the slab) <- (X diag(s)) R^-1 for upper R (32 x 32, ld
+// LDM, identity beyond nb) with D = 1 / diag(R); s == nullptr means no
+// scaling.  Two rows per thread held in registers, right-looking over the
+// columns: y_c *= D_c, then y_q -= y_c R(c,q) for q > c, R's row c read from
+// LDS as 16-byte broadcast pairs.  The scheduling barrier keeps each step's
+// loads inside the step (hoisted together they would not fit in
+// registers).
+__device__ __forceinline__ void rows_solve_upper(double *X, int RS, int nloc, int nb,
+                                              const double *R, const double *D,
+                                              const double *s, double *dummy) {
+  for (int base = 0; base < nloc; base += 2 * PNT) {
+    // rows past nloc compute on a copy of row nloc - 1 and store nothing
+    const int i0 = base + (int)threadIdx.x, i1 = i0 + PNT;
+    const bool w0 = i0 < nloc, w1 = i1 < nloc;
+    const int lr0 = min(i0, nloc - 1), lr1 = min(i1, nloc - 1);
+    double y0[PNB], y1[PNB];
+#pragma unroll
+    for (int c = 0; c < PNB; c++) {
+      // clamped, unconditional loads and selects: a branch per column makes
+      // the register allocator spill the rows
+      const int cc = min(c, nb - 1);
+      const double sc = s ? s[cc] : 1.0;
+      const double v0 = X[lr0 + cc * RS] * sc, v1 = X[lr1 + cc * RS] * sc;
+      y0[c] = (c < nb) ? v0 : 0.0;
+      y1[c] = (c < nb) ? v1 : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < PNB; c++) {
+      __builtin_amdgcn_sched_barrier(0);
+      const double dc = D[c];
+      y0[c] *= dc;
+      y1[c] *= dc;
+      const double *Rc = R + c * LDM;
+      int q = c + 1;
+      if (q & 1) {
+        if (q < PNB) {
+          const double r = Rc[q];
+          y0[q] -= y0[c] * r;
+          y1[q] -= y1[c] * r;
+        }
+        q++;
+      }
+#pragma unroll
+      for (; q < PNB; q += 2) {
+        const double2 r = *reinterpret_cast<const double2 *>(Rc + q);
+        y0[q] -= y0[c] * r.x;
+        y0[q + 1] -= y0[c] * r.y;
+        y1[q] -= y1[c] * r.x;
+        y1[q + 1] -= y1[c] * r.y;
+      }
+    }
+    // columns >= nb hold 0 and are stored to column nb - 1 first, so the
+    // store of y[nb - 1] comes last and wins; a row past nloc stores to a
+    // dummy LDS slot instead of a branch (a branch here spills)
+    double *d0 = w0 ? X + lr0 : dummy;
+    double *d1 = w1 ? X + lr1 : dummy;
+    const int st0 = w0 ? RS : 0, st1 = w1 ? RS : 0;
+#pragma unroll
+    for (int c = PNB - 1; c >= 0; c--) {
+      d1[min(c, nb - 1) * st1] = y1[c];
+      d0[min(c, nb - 1) * st0] = y0[c];
+    }
+  }
+}
+
+// one wave: sign-chosen LU of M (Q1 on entry, identity beyond nb) in place,
+// the column rule of lila-c/lila_dorgh2_3.c:8-21: at step k, S_k = -1 when
+// |1 - t| < |1 + t| for the diagonal t (column k negated), the pivot is
+// S_k t - 1, the column below is divided by it and the trailing block gets
+// the rank-1 update.  Lane c keeps column c in registers; the negation of
+// column k is carried as a sign (the steps before k are linear in column
+// k) and applied to its U entries at the end.  U (upper, diagonal the
+// pivots) and L (strict lower) overwrite M, S gets the signs and D the
+// reciprocal pivots.  Returns false when some |t| > 1 - 1e-10 (tau within
+// 1e-10 of 0 or 2).
+__device__ __forceinline__ bool wave_lu_sign32(double *M, double *S, double *D, int nb) {
+  const int lane = threadIdx.x & 63, c = lane & 31;
+  double a[PNB];
+#pragma unroll
+  for (int r = 0; r < PNB; r++) a[r] = M[r * LDM + c];
+  bool ok = true;
+  double mys = 1.0, myinv = 1.0;
+#pragma unroll
+  for (int k = 0; k < PNB; k++) {
+    if (k < nb) {
+      const double t = rdlane(a[k], k);
+      const bool flip = fabs(1.0 - t) < fabs(1.0 + t);
+      if (fabs(t) > 1.0 - 1e-10) ok = false;
+      const double sk = flip ? -1.0 : 1.0;
+      const double piv = sk * t - 1.0;
+      const double pinv = 1.0 / piv;
+      const double f = sk * pinv;
+      // U(k,c) for the trailing columns; finished columns c < k keep theirs
+      const double uk = (c > k) ? a[k] : 0.0;
+#pragma unroll
+      for (int r = k + 1; r < PNB; r++) {
+        const double l = rdlane(a[r], k) * f;  // L(r,k)
+        a[r] = (c == k) ? l : a[r] - l * uk;
+      }
+      if (c == k) {
+        a[k] = piv;
+        mys = sk;
+        myinv = pinv;
+      }
+    }
+  }
+  if (lane < PNB) {
+#pragma unroll
+    for (int r = 0; r < PNB; r++) M[r * LDM + c] = (r < c) ? a[r] * mys : a[r];
+    S[c] = mys;
+    D[c] = myinv;
+  }
+  wfence();
+  return ok;
+}
+
+// the fallback path out of line, so that its registers do not weigh on the
+// Cholesky-QR path's allocation
+__device__ __noinline__ void hh_fallback(const PanelArgs &a, double *slab, double *tail, int r0,
+                                         int nloc, unsigned &nbar) {
+  hh_factor<false>(a, slab, tail, r0, nloc, nbar);
+}
+
+__global__ __launch_bounds__(PNT) void panel_cqr_kernel(PanelArgs a) {
+  extern __shared__ double smem[];
+  const int nb = a.nb, RS = a.rs, G = a.G;
+  const int g = blockIdx.x, tid = threadIdx.x, wave = tid >> 6;
+  const int r0 = g * a.rp;
+  const int nloc = max(0, min(a.ml, r0 + a.rp) - r0);
+  double *X = smem;
+  double *tail = smem + slab_doubles(nb, RS);
+  double *M0 = tail, *M2 = M0 + PNB * LDM, *M3 = M2 + PNB * LDM;
+  double *Sv = M3 + PNB * LDM;  // 32 signs
+  double *D0 = Sv + PNB;        // 1 / diag of the current triangular factor
+  double *D2 = D0 + PNB;
+  int *flag = (int *)(D2 + PNB);  // [0] proceed
+  double *dummy = D2 + PNB + 1;   // target of discarded stores
+  double *G1 = a.scr + OFF_G1, *G2 = a.scr + OFF_G2, *pub = a.scr + OFF_PUB;
+  unsigned nbar = 0;
+
+  load_slab(a, X, r0, nloc);
+
+  // ---- pass 1: G1 = X^T X, R1 = chol(G1), X <- X R1^-1 ----
+  slab_gram(X, RS, nloc, nb, M0);
+  __syncthreads();
+  for (int t = tid; t < nb * nb; t += PNT) {
+    const int r = t / nb, c = t % nb;
+    if (r <= c) add_coh(&G1[r * PNB + c], M0[r * LDM + c]);
+  }
+  nbar++;
+  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t / PNB, c = t % PNB;
+    double v = 0.0;
+    if (r < nb && c < nb) { if (r <= c) v = ld_coh(&G1[r * PNB + c]); }
+    else if (r == c) v = 1.0;
+    M0[r * LDM + c] = v;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const bool ok = wave_chol32(M0, D0, nb);
+    if (tid == 0) flag[0] = ok ? 1 : 0;
+  }
+  __syncthreads();
+  if (flag[0]) {
+    rows_solve_upper(X, RS, nloc, nb, M0, D0, nullptr, dummy);
+    __syncthreads();
+    // ---- pass 2 ----
+    slab_gram(X, RS, nloc, nb, M2);
+    __syncthreads();
+    for (int t = tid; t < nb * nb; t += PNT) {
+      const int r = t / nb, c = t % nb;
+      if (r <= c) add_coh(&G2[r * PNB + c], M2[r * LDM + c]);
+    }
+    nbar++;
+    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    for (int t = tid; t < PNB * PNB; t += PNT) {
+      const int r = t / PNB, c = t % PNB;
+      double v = 0.0;
+      if (r < nb && c < nb) { if (r <= c) v = ld_coh(&G2[r * PNB + c]); }
+      else if (r == c) v = 1.0;
+      M2[r * LDM + c] = v;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const bool ok = wave_chol32(M2, D2, nb);
+      if (tid == 0) flag[0] = ok ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  if (flag[0]) {
+    rows_solve_upper(X, RS, nloc, nb, M2, D2, nullptr, dummy);  // X = Q
+    __syncthreads();
+    if (g == 0) {
+      // R = R2 R1 over R1 in M0 (upper), all threads, via registers
+      double racc[PNB * PNB / PNT];
+#pragma unroll
+      for (int u = 0; u < PNB * PNB / PNT; u++) {
+        const int t = tid + u * PNT, r = t / PNB, c = t % PNB;
+        double acc = 0.0;
+        if (r <= c)
+          for (int p = r; p <= c; p++) acc += M2[r * LDM + p] * M0[p * LDM + c];
+        racc[u] = acc;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < PNB * PNB / PNT; u++) {
+        const int t = tid + u * PNT, r = t / PNB, c = t % PNB;
+        M0[r * LDM + c] = racc[u];
+      }
+      // top nb x nb of Q into M3 (identity beyond nb)
+      for (int t = tid; t < PNB * PNB; t += PNT) {
+        const int r = t / PNB, c = t % PNB;
+        M3[r * LDM + c] = (r < nb && c < nb) ? X[r + c * RS] : (r == c ? 1.0 : 0.0);
+      }
+      __syncthreads();
+      if (wave == 0) {
+        const bool ok = wave_lu_sign32(M3, Sv, D2, nb);
+        // publish U (upper part of M3, identity beyond nb), S and the flag
+        const int l = tid & 63;
+        for (int t = l; t < PNB * PNB; t += 64) {
+          const int r = t / PNB, c = t % PNB;
+          st_coh(&pub[r * PNB + c], r <= c ? M3[r * LDM + c] : 0.0);
+        }
+        if (l < nb) st_coh(&pub[PNB * PNB + l], Sv[l]);
+        if (l == 0) st_coh(&pub[PNB * PNB + PNB], ok ? 1.0 : 0.0);
+      }
+    }
+    nbar++;
+    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    if (tid == 0) flag[0] = ld_coh(&pub[PNB * PNB + PNB]) != 0.0 ? 1 : 0;
+    __syncthreads();
+  }
+
+  if (!flag[0]) {
+    // fall back to the column-by-column Householder on the original panel
+    load_slab(a, X, r0, nloc);
+    hh_fallback(a, X, tail, r0, nloc, nbar);
+    exit_protocol(a);
+    return;
+  }
+
+  // ---- outputs: S, U (M2) and 1 / diag(U) in every workgroup ----
+  if (tid < nb) Sv[tid] = ld_coh(&pub[PNB * PNB + tid]);
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t / PNB, c = t % PNB;
+    M2[r * LDM + c] = ld_coh(&pub[r * PNB + c]);
+  }
+  __syncthreads();
+  if (tid < PNB) D2[tid] = 1.0 / M2[tid * LDM + tid];
+  // explicit panel Q = Q S
+  if (a.Q) store_slab(X, RS, nb, nloc, a.Q + r0, a.ldq, Sv);
+  __syncthreads();
+  // V2 = (Q2 S) U^-1 (the top rows of workgroup 0 are replaced below)
+  rows_solve_upper(X, RS, nloc, nb, M2, D2, Sv, dummy);
+  __syncthreads();
+  if (g == 0) {
+    // top nb rows: R' = S R2 R1 on and above the diagonal, V1 = L below
+    for (int t = tid; t < nb * nb; t += PNT) {
+      const int r = t % nb, c = t / nb;
+      X[r + c * RS] = (r <= c) ? Sv[r] * M0[r * LDM + c] : M3[r * LDM + c];
+    }
+    // T = -U V1^-T, one row of T per thread: T V1^T = -U, so with t = row
+    // r of T, right-looking over p: t_c -= t_p V1(c,p) for c > p
+    if (tid < nb) {
+      const int r = tid;
+      double t[PNB];
+#pragma unroll
+      for (int c = 0; c < PNB; c++) {
+        const double u = -M2[r * LDM + c];  // 0 beyond nb (U is padded)
+        t[c] = (c >= r) ? u : 0.0;
+      }
+#pragma unroll
+      for (int p = 0; p < PNB; p++) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = p + 1; c < PNB; c++) t[c] -= t[p] * M3[c * LDM + p];
+      }
+      // t_c = 0 for c >= nb; as above, column nb - 1 is written last
+#pragma unroll
+      for (int c = PNB - 1; c >= 0; c--)
+        if (c >= r) a.T[(size_t)r + (size_t)min(c, nb - 1) * a.ldt] = t[c];
+    }
+  }
+  __syncthreads();
+  store_slab(X, RS, nb, nloc, a.A + r0, a.lda, nullptr);
+  exit_protocol(a);
+}
+
+size_t panel_smem_bytes(int nb, int rs) {
+  return sizeof(double) * ((((size_t)nb * rs + 1) & ~(size_t)1) + LDS_TAIL);
 }
 
 }  // namespace
 
 // persistent scratch size (doubles) the runtime allocates and zeroes once
-extern "C" size_t ld_panel_scratch_doubles(void) { return 3 * SLOT + PNB * PNB; }
+extern "C" size_t ld_panel_scratch_doubles(void) { return SCRATCH; }
 
-extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int ldt,
-                            double *Q, int ldq) {
+// mode: 0 column-by-column Householder, 1 Cholesky-QR2 + reconstruction
+// (with the column-by-column fallback inside)
+static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
+                         int ldq) {
   if (nb <= 0 || ml <= 0) return;
   if (nb > PNB) {
     ld_fatal("panel width above 32 reached the fused panel kernel");
     return;
   }
-  int target = ld_tune()->panel_rows_per_wg;
+  // slab rows per workgroup: as many as LDS holds next to the fixed part,
+  // with an odd column stride (LDS bank spread)
+  int rmax = LDS_SLAB_DOUBLES / nb;
+  if (!(rmax & 1)) rmax--;
+  int target = (mode == 1) ? ld_tune()->panel_rows_cqr : ld_tune()->panel_rows_per_wg;
   if (target < nb) target = nb;
-  if (target > RMAX) target = RMAX;
+  if (target > rmax) target = rmax;
   int G = (ml + target - 1) / target;
   if (G > 256) G = 256;
   int rp = (ml + G - 1) / G;
   if (rp < nb) rp = nb;
   G = (ml + rp - 1) / rp;
-  if (rp > RMAX) {
-    ld_fatal("panel taller than 256 x 480 rows is not supported by the fused panel kernel");
+  if (rp > rmax) {
+    ld_fatal("panel taller than 256 slabs of LDS rows reached the fused panel kernel");
     return;
   }
+  const int rs = rp | 1;  // odd LDS column stride
   // Co-residency of all G workgroups is what the in-kernel barrier needs.
   // It is checked once from the occupancy calculator (one workgroup per CU
   // at the largest LDS size, G <= CU count); a plain launch then keeps all
@@ -392,16 +876,22 @@ extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int l
   // cooperative launch API gives the same guarantee at a higher launch cost
   // and is used when LILA_PANEL_COOP=1 or when the check fails.
   static int attr_set = 0, ncu = 0, coop = 0;
+  const size_t maxb = LDS_DYN_BYTES;
   if (!attr_set) {
-    hipFuncSetAttribute((const void *)panel_qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)panel_smem_bytes(PNB, RMAX));
-    int dev = 0, per_cu = 0;
+    hipFuncSetAttribute((const void *)panel_hh_kernel<false>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
+    hipFuncSetAttribute((const void *)panel_hh_kernel<true>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
+    hipFuncSetAttribute((const void *)panel_cqr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)maxb);
+    int dev = 0, per_cu = 0, per_cu2 = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)panel_qr_kernel, PNT,
-                                                 panel_smem_bytes(PNB, RMAX));
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)panel_hh_kernel<false>, PNT,
+                                                 maxb);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, (const void *)panel_cqr_kernel, PNT, maxb);
     const char *e = getenv("LILA_PANEL_COOP");
-    coop = (e && atoi(e) == 1) || per_cu < 1 || ncu < 1;
+    coop = (e && atoi(e) == 1) || per_cu < 1 || per_cu2 < 1 || ncu < 1;
     attr_set = 1;
   }
   if (ncu > 0 && G > ncu) {
@@ -410,21 +900,31 @@ extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int l
   }
   PanelArgs pa;
   pa.ml = ml; pa.nb = nb; pa.A = A; pa.lda = lda; pa.T = T; pa.ldt = ldt;
-  pa.Q = Q; pa.ldq = ldq; pa.G = G; pa.rp = rp;
-  pa.red = ld_panel_scratch();
+  pa.Q = Q; pa.ldq = ldq; pa.G = G; pa.rp = rp; pa.rs = rs;
+  pa.scr = ld_panel_scratch();
   int *flags = ld_device_flags();
   pa.cnt = (unsigned *)(flags + 8);
   pa.err = flags;
-  void *args[] = {&pa};
-  const int ph = ld_prof_begin("panel_qr_kernel");
-  hipError_t e;
-  if (coop) {
-    e = hipLaunchCooperativeKernel((const void *)panel_qr_kernel, dim3(G), dim3(PNT), args,
-                                   (unsigned)panel_smem_bytes(nb, rp), (hipStream_t)ld_stream());
+  const void *fn;
+  const char *cls;
+  if (G == 1) {
+    fn = (const void *)panel_hh_kernel<true>;
+    cls = "panel_hh_kernel<true>";
+  } else if (mode == 1) {
+    fn = (const void *)panel_cqr_kernel;
+    cls = "panel_cqr_kernel";
   } else {
-    hipLaunchKernelGGL(panel_qr_kernel, dim3(G), dim3(PNT), panel_smem_bytes(nb, rp),
-                       (hipStream_t)ld_stream(), pa);
-    e = hipGetLastError();
+    fn = (const void *)panel_hh_kernel<false>;
+    cls = "panel_hh_kernel<false>";
+  }
+  void *args[] = {&pa};
+  const size_t shb = panel_smem_bytes(nb, rs);
+  const int ph = ld_prof_begin(cls);
+  hipError_t e;
+  if (coop && G > 1) {
+    e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(PNT), args, (unsigned)shb, (hipStream_t)ld_stream());
+  } else {
+    e = hipLaunchKernel(fn, dim3(G), dim3(PNT), args, shb, (hipStream_t)ld_stream());
   }
   // algorithmic traffic: read the panel, write V/R (+ Q), write T
   ld_prof_end(ph, 2.0 * ml * nb * nb + (Q ? 2.0 * ml * nb * nb : 0.0),
@@ -434,4 +934,14 @@ extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int l
             hipGetErrorString(e));
     ld_set_info(-1);
   }
+}
+
+extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
+                            int ldq) {
+  panel_launch(ld_tune()->panel_mode, ml, nb, A, lda, T, ldt, Q, ldq);
+}
+
+extern "C" void ld_panel_qr_exact(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
+                                  int ldq) {
+  panel_launch(0, ml, nb, A, lda, T, ldt, Q, ldq);
 }

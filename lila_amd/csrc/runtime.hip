@@ -38,7 +38,7 @@ struct State {
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
-  ld_tune_t tune = {384, 512, 64, 64, 128, 256};
+  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1};
 };
 State g;
 }  // namespace
@@ -60,7 +60,7 @@ int ld_init(void) {
     if (hipMemset(g.pscratch, 0, pb) != hipSuccess) return -1;
   }
   const char *e = getenv("LILA_PANEL_ROWS");
-  if (e) g.tune.panel_rows_per_wg = atoi(e);
+  if (e) g.tune.panel_rows_per_wg = g.tune.panel_rows_cqr = atoi(e);
   e = getenv("LILA_GEMM_MIN_WGS");
   if (e) g.tune.gemm_min_wgs = atoi(e);
   e = getenv("LILA_GEMM_TILE");  // "BMxBN", e.g. 128x64
@@ -74,6 +74,8 @@ int ld_init(void) {
   }
   e = getenv("LILA_GEMM_SMALL_TILES");
   if (e) g.tune.gemm_small_tiles = atoi(e);
+  e = getenv("LILA_PANEL_MODE");
+  if (e) g.tune.panel_mode = atoi(e) ? 1 : 0;
   g.inited = 1;
   return 0;
 }
@@ -319,7 +321,14 @@ void ld_prof_end(int h, double flops, double bytes) {
   hipEventRecord(prec[h].b, g.stream);
 }
 
-// ---- exported (include/lila.h extension) ----
+// ---- exported (include/lila.h extensions) ----
+int lila_set_panel_mode(int mode) {
+  if (ld_init() != 0) return -1;
+  const int old = g.tune.panel_mode;
+  g.tune.panel_mode = mode ? 1 : 0;
+  return old;
+}
+
 int lila_prof_enable(int on) {
   if (ld_init() != 0) return -1;
   hipStreamSynchronize(g.stream);
