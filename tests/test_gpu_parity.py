@@ -297,6 +297,65 @@ def test_dorgh2_3_and_getrf_b(lila, ora):
     assert rel(to_host(Ad2), Ao2) < tol and rel(to_host(Qd2), Qo2) < tol and rel(to_host(Ud), Uo) < tol
 
 
+def _orthonormal(ora, m, n, seed):
+    A = ora.fill_normal(seed, m, n)
+    Q = A.copy(order="F")
+    R = np.zeros((n, n), order="F")
+    ora.call("dsyrk", "U", "T", n, m, 1.0, Q, m, 0.0, R, n)
+    assert ora.call("dpotrf_u", n, R, n) == 0
+    ora.call("dtrsm", "R", "U", "N", "N", m, n, 1.0, R, n, Q, m)
+    # second pass for orthogonality to rounding
+    R2 = np.zeros((n, n), order="F")
+    ora.call("dsyrk", "U", "T", n, m, 1.0, Q, m, 0.0, R2, n)
+    assert ora.call("dpotrf_u", n, R2, n) == 0
+    ora.call("dtrsm", "R", "U", "N", "N", m, n, 1.0, R2, n, Q, m)
+    return Q, np.asfortranarray(np.triu(R2) @ np.triu(R))
+
+
+@pytest.mark.parametrize("m,n,i,j,mt", [(300, 24, 0, 0, 32), (420, 16, 5, 44, 20), (500, 40, 11, 64, 64)])
+def test_dorgh2_global(lila, ora, m, n, i, j, mt):
+    say("dorgh2", m, n, i, j, mt)
+    Qb, R = _orthonormal(ora, m - j, n, 21)
+    A = ora.fill_uniform(3, m, j + n, -1.0, 1.0)
+    A[j:j + n, j:j + n] = R
+    A[j + n:, j:j + n] = Qb[n:]
+    T = ora.fill_uniform(4, mt, j + n, -1.0, 1.0)
+    T[j % mt:j % mt + n, j:j + n] = Qb[:n]
+    Q = ora.fill_uniform(5, m, j + n, -1.0, 1.0)
+    Q[j:, j:j + n] = Qb
+    So = np.zeros(j + n)
+    Ao, To, Qo = A.copy(order="F"), T.copy(order="F"), Q.copy(order="F")
+    ora.call("lila_dorgh2", m, n, i, j, mt, Ao, m, To, mt, Qo, m, None, 0, So)
+    Ad, Td, Qd = to_dev(A), to_dev(T), to_dev(Q)
+    Sd = torch.zeros(j + n, dtype=torch.float64, device="cuda")
+    assert lila.call("lila_dorgh2", m, n, i, j, mt, Ad, m, Td, mt, Qd, m, None, 0, Sd) == 0
+    sync_ok(lila)
+    assert np.array_equal(Sd.cpu().numpy()[j:], So[j:])
+    tol = 200 * n * U64
+    assert rel(to_host(Ad), Ao) < tol and rel(to_host(Td), To) < tol and rel(to_host(Qd), Qo) < tol
+
+
+@pytest.mark.parametrize("m,n,i,mt", [(400, 40, 0, 16), (600, 70, 9, 32), (500, 50, 3, 64)])
+def test_dorghr_blocked(lila, ora, m, n, i, mt):
+    say("dorghr", m, n, i, mt)
+    Qb, R = _orthonormal(ora, m - i, n, 23)
+    A = np.zeros((m, i + n), order="F")
+    A[i:i + n, i:] = R
+    Q = np.zeros((m, i + n), order="F")
+    Q[i:, i:] = Qb
+    T = np.zeros((mt, i + n), order="F")
+    So = np.zeros(i + n, dtype=np.int32)
+    Ao, To, Qo = A.copy(order="F"), T.copy(order="F"), Q.copy(order="F")
+    ora.call("lila_dorghr", m, n, i, mt, Ao, m, To, mt, Qo, m, None, 0, So)
+    Ad, Td, Qd = to_dev(A), to_dev(T), to_dev(Q)
+    Sd = torch.zeros(i + n, dtype=torch.int32, device="cuda")
+    assert lila.call("lila_dorghr", m, n, i, mt, Ad, m, Td, mt, Qd, m, None, 0, Sd) == 0
+    sync_ok(lila)
+    assert np.array_equal(Sd.cpu().numpy()[i:], So[i:])
+    tol = 500 * n * U64
+    assert rel(to_host(Ad), Ao) < tol and rel(to_host(Td), To) < tol and rel(to_host(Qd), Qo) < tol
+
+
 def test_dgels_qr3(lila, ora):
     m, n, k = 800, 60, 5
     say("dgels_qr3")

@@ -311,6 +311,63 @@ def test_dorgh2_3_matches_dorhr_col(ora, m, n):
     assert rel(np.triu(Ao[:n]), S[:, None] * R) == 0.0
 
 
+@pytest.mark.parametrize("m,n,i,j,mt", [(200, 24, 0, 0, 32), (260, 16, 5, 44, 20), (300, 40, 11, 64, 64)])
+def test_dorgh2_global_matches_dorhr_col(ora, m, n, i, j, mt):
+    """lila_dorgh2 (global index, T block at row j % mt, Q columns flipped on
+    rows [i, m)) reduces to dorhr_col on the block A(j:m, j:j+n)"""
+    Qb, R = orthonormal(ora, m - j, n, 17)
+    A = ora.fill_uniform(3, m, j + n, -1.0, 1.0)
+    A[j:j + n, j:j + n] = R
+    A[j + n:, j:j + n] = Qb[n:]
+    T = ora.fill_uniform(4, mt, j + n, -1.0, 1.0)
+    T[j % mt:j % mt + n, j:j + n] = Qb[:n]
+    Q = ora.fill_uniform(5, m, j + n, -1.0, 1.0)
+    Q[j:, j:j + n] = Qb
+    S = np.zeros(j + n)
+    Ao, To, Qo = A.copy(order="F"), T.copy(order="F"), Q.copy(order="F")
+    ora.call("lila_dorgh2", m, n, i, j, mt, Ao, m, To, mt, Qo, m, None, 0, S)
+    Y, Tl, D = lapack_dorhr_col(Qb)
+    assert np.array_equal(S[j:], D)
+    tol = 50 * n * U64
+    assert rel(np.tril(Ao[j:, j:j + n], -1), np.tril(Y, -1)) < tol
+    assert rel(np.triu(To[j % mt:j % mt + n, j:j + n]), np.triu(Tl)) < tol
+    assert np.array_equal(np.triu(Ao[j:j + n, j:j + n]), D[:, None] * R)
+    # columns j.. of Q flipped by S on rows [i, m) only; the rest untouched
+    assert np.array_equal(Qo[i:, j:], Q[i:, j:] * D)
+    assert np.array_equal(Qo[:i], Q[:i]) and np.array_equal(Qo[:, :j], Q[:, :j])
+    assert np.array_equal(Ao[:, :j], A[:, :j]) and np.array_equal(Ao[:j], A[:j])
+
+
+@pytest.mark.parametrize("m,n,i,mt", [(240, 40, 0, 16), (300, 70, 9, 32), (200, 50, 3, 64)])
+def test_dorghr_blocked_closed_forms(ora, m, n, i, mt):
+    """blocked reconstruction (c3/lila.h:43, int S): with Q orthonormal and R
+    on top, the mt-blocked (V, T) reproduce Q S and S R, and every block's
+    sign follows the rule of its own diagonal"""
+    Qb, R = orthonormal(ora, m - i, n, 19)
+    A = np.zeros((m, i + n), order="F")
+    A[i:i + n, i:] = R
+    Q = np.zeros((m, i + n), order="F")
+    Q[i:, i:] = Qb
+    T = np.zeros((mt, i + n), order="F")
+    S = np.zeros(i + n, dtype=np.int32)
+    Ao, To, Qo = A.copy(order="F"), T.copy(order="F"), Q.copy(order="F")
+    ora.call("lila_dorghr", m, n, i, mt, Ao, m, To, mt, Qo, m, None, 0, S)
+    Sd = S[i:].astype(np.float64)
+    assert set(np.unique(Sd)) <= {-1.0, 1.0}
+    tol = 100 * n * U64
+    assert np.array_equal(Qo[i:, i:], Qb * Sd)
+    # (V, T) applied to the matrix Q R give [S R; 0]
+    As = np.zeros((m, i + n), order="F")
+    As[i:, i:] = Qb @ R
+    assert ora.call("test_r_repres_2", m, n, n, i, mt, As, m, Ao, m, To, mt) < tol
+    assert rel(np.triu(Ao[i:i + n, i:]), Sd[:, None] * R) < tol
+    # single block: the blocked driver is dorhr_col
+    if n <= mt - i % mt:
+        Y, Tl, D = lapack_dorhr_col(Qb)
+        assert np.array_equal(Sd, D)
+        assert rel(np.tril(Ao[i:, i:], -1), np.tril(Y, -1)) < tol
+
+
 @pytest.mark.parametrize("m,n,ii,mt", [(300, 40, 0, 64), (300, 96, 7, 32), (400, 130, 3, 50)])
 def test_mt_hr_closed_forms(ora, m, n, ii, mt):
     A = ora.fill_normal(5, m, n + ii)
