@@ -75,7 +75,7 @@ def cpu_baseline(m_target, mt, nx, budget_s):
     t = run(nn)
     rate = vrtq_flops(nn, nn) / t
     # pick the size whose run takes about budget_s at the measured rate
-    while nn < min(8192, m_target):
+    while nn < m_target:
         nxt = int(nn * 1.5) // 64 * 64
         if vrtq_flops(nxt, nxt) / rate > budget_s:
             break
