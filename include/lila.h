@@ -162,6 +162,11 @@ int lila_set_panel_mode(int mode);
  * kernel class, "name|launches|algorithmic_flops|algorithmic_bytes|ms". */
 int lila_prof_enable(int on);
 int lila_prof_read(char *buf, int buflen);
+/* Record only launches of this kernel class (its exact name as printed by
+ * lila_prof_read); NULL or "" records every class.  Each recorded launch
+ * adds two event records to the stream, so timing runs filter to the class
+ * they report. */
+int lila_prof_only(const char *kernel_class);
 
 /* Device generators of the test inputs (splitmix64 stream; Box-Muller). */
 int lila_fill_uniform(unsigned long long seed, int m, int n, double *A, int lda, double lo,

@@ -212,17 +212,38 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
   }
 }
 
-__global__ void splitk_reduce_kernel(int M, int N, int S, double alpha, double beta,
-                                     const double *__restrict__ P,
-                                     double *__restrict__ C, int ldc, int zero_rows,
-                                     int c_upper) {
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t tot = (size_t)M * N;
-  if (idx >= tot) return;
-  int i = (int)(idx % M), j = (int)(idx / M);
-  if (c_upper && i > j) return;
+// 64 outputs per workgroup; wave q sums partials q, q + 4, ... with two
+// accumulators (several loads in flight), the four wave sums are combined in
+// a fixed order: deterministic for a given split count
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int S, double alpha,
+                                                            double beta,
+                                                            const double *__restrict__ P,
+                                                            double *__restrict__ C, int ldc,
+                                                            int zero_rows, int c_upper) {
+  __shared__ double red[4][64];
+  const int l = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const size_t tot = (size_t)M * N;
+  const size_t idx = (size_t)blockIdx.x * 64 + l;
   double s = 0.0;
-  for (int z = 0; z < S; z++) s += P[(size_t)z * tot + idx];
+  if (idx < tot) {
+    const double *p = P + idx;
+    double s0 = 0.0, s1 = 0.0;
+    int z = q;
+    for (; z + 12 < S; z += 16) {
+      const double a0 = p[(size_t)z * tot], a1 = p[(size_t)(z + 4) * tot];
+      const double a2 = p[(size_t)(z + 8) * tot], a3 = p[(size_t)(z + 12) * tot];
+      s0 += a0 + a2;
+      s1 += a1 + a3;
+    }
+    for (; z < S; z += 4) s0 += p[(size_t)z * tot];
+    s = s0 + s1;
+  }
+  red[q][l] = s;
+  __syncthreads();
+  if (q != 0 || idx >= tot) return;
+  const int i = (int)(idx % M), j = (int)(idx / M);
+  if (c_upper && i > j) return;
+  s = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
   double *c = &C[(size_t)i + (size_t)j * ldc];
   double o = alpha * s;
   if (beta != 0.0 && i >= zero_rows) o += beta * (*c);
@@ -275,9 +296,11 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
   }
   int splits = 1;
   const int target = tu->gemm_min_wgs;
-  if (tiles < target && K >= 512) {
+  if (tiles < target && K >= 256) {
     splits = (int)((target + tiles - 1) / tiles);
-    int maxs = K / 256;
+    // down to 4 k-steps of BK per split: a long-K product with a one-tile
+    // output then spreads over up to K/64 workgroups
+    int maxs = K / 64;
     if (splits > maxs) splits = maxs;
     // cap partial storage at 64 Mi doubles
     long cap = (64L << 20) / ((long)M * N);
@@ -303,7 +326,7 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
   if (splits > 1) {
     size_t tot = (size_t)M * N;
     const int ph = ld_prof_begin("splitk_reduce_kernel");
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((tot + 63) / 64)), dim3(256), 0, st,
                        M, N, splits, g->alpha, g->beta, P, g->C, g->ldc, g->zero_rows,
                        g->c_upper);
     ld_prof_end(ph, (double)splits * tot, 8.0 * ((double)splits * tot + 2.0 * tot));

@@ -250,6 +250,7 @@ struct ProfCls {
   double flops, bytes, ms;
 };
 int prof_on = 0;
+char prof_only[96] = "";  // when set, only classes with this exact name are recorded
 ProfRec *prec = nullptr;
 int nrec = 0, caprec = 0;
 ProfCls pcls[64];
@@ -288,6 +289,7 @@ extern "C" {
 
 int ld_prof_begin(const char *name) {
   if (!prof_on) return -1;
+  if (prof_only[0] && strcmp(prof_only, name) != 0) return -1;
   int c = 0;
   for (; c < ncls; c++)
     if (strcmp(pcls[c].name, name) == 0) break;
@@ -335,6 +337,15 @@ int lila_prof_enable(int on) {
   prof_flush();
   if (on) ncls = 0;
   prof_on = on;
+  return 0;
+}
+
+// restrict recording to one kernel class (NULL or "" = all classes)
+int lila_prof_only(const char *kernel_class) {
+  if (ld_init() != 0) return -1;
+  hipStreamSynchronize(g.stream);
+  prof_flush();
+  snprintf(prof_only, sizeof prof_only, "%s", kernel_class ? kernel_class : "");
   return 0;
 }
 
