@@ -125,6 +125,76 @@ int lila_query_dormqrf_w03(int m, int n, int k, int i, int j, int mt, double *A,
 int lila_query_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A, int lda, double *Q,
                                         int ldq, double *T, int ldt, double *work, int lwork);
 
+/* lila-c/lila.h:19, :21, :22 -- Q^T applied with LAPACK dormqr semantics:
+ * w00/z00 rebuild tau = 2/(1 + ||v||^2) from the stored vectors (C = A's
+ * columns j.. or B); z02 applies one block with the full-layout T at T(i,i).
+ * Note z00's order (A, B, T) and z02's (A, T, B), as in the reference. */
+int lila_dormqrf_w00(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                     int ldt, double *work, int lwork);
+int lila_dormqrf_z00(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *B,
+                     int ldb, double *T, int ldt, double *work, int lwork);
+int lila_dormqrf_z02(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                     int ldt, double *B, int ldb, double *work, int lwork);
+
+/* lila-c/lila.h:74-81 -- V, R and T only (no Q); recursion leaf when n < nx */
+int lila_dgeqrf_v03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n, int i,
+                           int mt, double *A, int lda, double *T, int ldt, double *work, int lwork);
+int lila_dgeqrf_v03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *work, int lwork);
+int lila_dgeqrf_v03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                       int ldt, double *work, int lwork);
+int lila_dgeqr2_v03(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                    int ldt, double *work, int lwork);
+int lila_dgeqrf_v03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                          double *T, int ldt, double *work, int lwork);
+int lila_dgeqrf_v03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                         double *work, int lwork);
+int lila_dgeqr2_v03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                      double *work, int lwork);
+int lila_dgeqr2_v03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                      double *work, int lwork);
+
+/* lila-c/lila.h:83-90 -- explicit Q from the stored (V, T) of a v03 pass */
+int lila_dgeqrf_q03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n, int i,
+                           int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                           double *work, int lwork);
+int lila_dgeqrf_q03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqrf_q03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                          double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqrf_q03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                       int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqrf_q03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                         double *Q, int ldq, double *work, int lwork);
+int lila_dgeqr2_q03(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                    int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqr2_q03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                      double *Q, int ldq, double *work, int lwork);
+int lila_dgeqr2_q03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                      double *Q, int ldq, double *work, int lwork);
+
+/* lila-c/lila.h:95 -- recursive QR on LAPACK kernels; T holds tau (T + i is
+ * the tau of the columns from i) */
+int lila_dgeqrf_ker_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+
+/* lila-c/lila.h:97-99 -- the recursion with lila_param[4] choosing VRTQ (0),
+ * VRT (1) or Q from (V, T) (2) */
+int lila_dgeqrf_qr2_recursive2(int *lila_param, int m, int n, int i, int mt, int nb1, int nb2,
+                               double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                               double *work, int lwork);
+int lila_dgeqrf_qr2_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int lila_dgeqrf_qr2(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                    int ldt, double *Q, int ldq, double *work, int lwork);
+
+/* lila-c/lila.h:101-102 -- QR of k factored columns extended by n new ones
+ * (Block-Arnoldi); the LAPACK form keeps tau (k + n entries) in T */
+int lila_dgeqrf_w03_appendcols(int *lila_param, int m, int k, int n, int mt, double *A, int lda,
+                               double *Q, int ldq, double *T, int ldt, double *work, int lwork);
+int lila_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A, int lda, double *Q,
+                                  int ldq, double *T, int ldt, double *work, int lwork);
+
 /* lila-c/lila.h:92-93 */
 int lila_dT2tau_w03(int m, int n, int i, int mt, double *T, int ldt, double *tau);
 int lila_dV2tau_w03(int m, int n, int i, int mt, double *A, int lda, double *tau);

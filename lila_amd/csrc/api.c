@@ -214,6 +214,182 @@ int lila_dV2tau_w03(int m, int n, int i, int mt, double *A, int lda, double *tau
     return api_end();
 }
 
+/* ---- VRT only (v03), Q from (V, T) (q03), append-columns, LAPACK-kernel
+ * and qr2 families ------------------------------------------------------- */
+
+#define VRT_ENTRY(NAME, CALL)                                                               \
+    int NAME {                                                                              \
+        (void)work; (void)lwork;                                                            \
+        BEGIN();                                                                            \
+        stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);           \
+        CALL;                                                                               \
+        stage_out(&a, 1); stage_out(&t, 1);                                                 \
+        return api_end();                                                                   \
+    }
+
+VRT_ENTRY(lila_dgeqrf_v03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                    double *T, int ldt, double *work, int lwork),
+          dv_dgeqrf_v03_recursive(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt))
+VRT_ENTRY(lila_dgeqrf_v03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                             int ldt, double *work, int lwork),
+          dv_dgeqrf_v03_mt(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt))
+VRT_ENTRY(lila_dgeqr2_v03(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                          int ldt, double *work, int lwork),
+          dv_dgeqr2_v03(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt))
+VRT_ENTRY(lila_dgeqrf_v03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                double *T, int ldt, double *work, int lwork),
+          dv_dgeqrf_v03_mt_hh(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt))
+VRT_ENTRY(lila_dgeqrf_v03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                               double *work, int lwork),
+          dv_dgeqrf_v03_mt_l(m, n, i, mt, a.dev, lda, t.dev, ldt))
+VRT_ENTRY(lila_dgeqr2_v03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                            double *work, int lwork),
+          dv_dgeqr2_v03_3(m, n, i, mt, a.dev, lda, t.dev, ldt))
+VRT_ENTRY(lila_dgeqr2_v03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                            double *work, int lwork),
+          dv_dgeqr2_v03_3(m, n, i, mt, a.dev, lda, t.dev, ldt))
+
+int lila_dgeqrf_v03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n, int i,
+                           int mt, double *A, int lda, double *T, int ldt, double *work, int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
+    dv_dgeqrf_v03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt);
+    stage_out(&a, 1); stage_out(&t, 1);
+    return api_end();
+}
+
+/* Q builds read A and T and write Q */
+#define Q03_ENTRY(NAME, CALL)                                                               \
+    int NAME {                                                                              \
+        (void)work; (void)lwork;                                                            \
+        BEGIN();                                                                            \
+        stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);           \
+        stage_t q = stage_in(Q, ldq, m, i + n);                                             \
+        CALL;                                                                               \
+        stage_out(&a, 0); stage_out(&t, 0); stage_out(&q, 1);                               \
+        return api_end();                                                                   \
+    }
+
+Q03_ENTRY(lila_dgeqrf_q03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                    double *T, int ldt, double *Q, int ldq, double *work, int lwork),
+          dv_dgeqrf_q03_recursive(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+Q03_ENTRY(lila_dgeqrf_q03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                             int ldt, double *Q, int ldq, double *work, int lwork),
+          dv_dgeqrf_q03_mt(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+Q03_ENTRY(lila_dgeqrf_q03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                double *T, int ldt, double *Q, int ldq, double *work, int lwork),
+          dv_dgeqrf_q03_mt_hh(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+Q03_ENTRY(lila_dgeqrf_q03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                               double *Q, int ldq, double *work, int lwork),
+          dv_dgeqrf_q03_mt_l(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+Q03_ENTRY(lila_dgeqr2_q03(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                          int ldt, double *Q, int ldq, double *work, int lwork),
+          dv_dgeqr2_q03(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+Q03_ENTRY(lila_dgeqr2_q03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                            double *Q, int ldq, double *work, int lwork),
+          dv_dgeqr2_q03_l(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+Q03_ENTRY(lila_dgeqr2_q03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                            double *Q, int ldq, double *work, int lwork),
+          dv_dgeqr2_q03_3(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+
+int lila_dgeqrf_q03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n, int i,
+                           int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                           double *work, int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
+    stage_t q = stage_in(Q, ldq, m, i + n);
+    dv_dgeqrf_q03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev,
+                         ldq);
+    stage_out(&a, 0); stage_out(&t, 0); stage_out(&q, 1);
+    return api_end();
+}
+
+int lila_dgeqrf_w03_appendcols(int *lila_param, int m, int k, int n, int mt, double *A, int lda,
+                               double *Q, int ldq, double *T, int ldt, double *work, int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, k + n), t = stage_in(T, ldt, ldt, k + n);
+    stage_t q = stage_in(Q, ldq, m, k + n);
+    dv_dgeqrf_w03_appendcols(lila_param, m, k, n, mt, a.dev, lda, q.dev, ldq, t.dev, ldt);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
+    return api_end();
+}
+
+int lila_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A, int lda, double *Q, int ldq,
+                                  double *T, int ldt, double *work, int lwork) {
+    (void)mt; (void)ldt; (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, k + n), t = stage_in(T, 1, 1, k + n);
+    stage_t q = stage_in(Q, ldq, m, k + n);
+    dv_dgeqrf_LAPACK_appendcols(m, k, n, a.dev, lda, q.dev, ldq, t.dev);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
+    return api_end();
+}
+
+int lila_dgeqrf_ker_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *Q, int ldq, double *work, int lwork) {
+    (void)mt; (void)ldt; (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, 1, 1, i + n);
+    stage_t q = stage_in(Q, ldq, m, i + n);
+    dv_dgeqrf_ker_recursive(lila_param, m, n, i, a.dev, lda, t.dev, q.dev, ldq);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
+    return api_end();
+}
+
+int lila_dormqrf_w00(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T, int ldt,
+                     double *work, int lwork) {
+    (void)mt; (void)T; (void)ldt; (void)work; (void)lwork;
+    BEGIN();
+    int cols = (j + n > i + k) ? j + n : i + k;
+    stage_t a = stage_in(A, lda, m, cols);
+    dv_dormqrf_tau_from_v(m, n, k, i, a.dev, lda, a.dev + i + (size_t)j * lda, lda);
+    stage_out(&a, 1);
+    return api_end();
+}
+
+int lila_dormqrf_z00(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *B, int ldb,
+                     double *T, int ldt, double *work, int lwork) {
+    (void)mt; (void)T; (void)ldt; (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + k), b = stage_in(B, ldb, m, j + n);
+    dv_dormqrf_tau_from_v(m, n, k, i, a.dev, lda, b.dev + i + (size_t)j * ldb, ldb);
+    stage_out(&a, 0); stage_out(&b, 1);
+    return api_end();
+}
+
+int lila_dormqrf_z02(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T, int ldt,
+                     double *B, int ldb, double *work, int lwork) {
+    (void)mt; (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + k), t = stage_in(T, ldt, i + k, i + k);
+    stage_t b = stage_in(B, ldb, m, j + n);
+    dv_dormqrf_z02(m, n, k, i, j, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    stage_out(&a, 0); stage_out(&t, 0); stage_out(&b, 1);
+    return api_end();
+}
+
+LEAF_ENTRY(lila_dgeqrf_qr2(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                           int ldt, double *Q, int ldq, double *work, int lwork),
+           dv_dgeqrf_qr2(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+LEAF_ENTRY(lila_dgeqrf_qr2_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                     double *T, int ldt, double *Q, int ldq, double *work, int lwork),
+           dv_dgeqrf_qr2_recursive(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq))
+
+int lila_dgeqrf_qr2_recursive2(int *lila_param, int m, int n, int i, int mt, int nb1, int nb2, double *A,
+                               int lda, double *T, int ldt, double *Q, int ldq, double *work,
+                               int lwork) {
+    (void)work; (void)lwork;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
+    stage_t q = stage_in(Q, ldq, m, i + n);
+    dv_dgeqrf_qr2_recursive2(lila_param, m, n, i, mt, nb1, nb2, a.dev, lda, t.dev, ldt, q.dev, ldq);
+    stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
+    return api_end();
+}
+
 /* ---- Householder reconstruction ----------------------------------------- */
 
 int lila_dorgh2_3(int m, int n, double *A, int lda, double *T, int ldt, double *Q, int ldq,

@@ -422,6 +422,333 @@ int dv_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, doubl
     return 0;
 }
 
+/* ---- VRT only (v03), Q from (V, T) (q03), append-columns, LAPACK-kernel
+ * and qr2 families (lila-c/lila.h:19-22, :74-90, :95-102) ------------------ */
+
+/* C <- (I - V T V^T) C for one block (the non-transposed application) */
+static void apply_block_q(int ml, int n, int vb, const double *V, int ldv, const double *Tb, int ldt,
+                          double *C, int ldc) {
+    if (vb <= 0 || n <= 0) return;
+    size_t mark = ld_ws_mark();
+    double *W1 = wsd((size_t)vb * n), *W2 = wsd((size_t)vb * n);
+    gemm(T_, N_, vb, n, ml, 1.0, V, ldv, LD_VUNIT, C, ldc, LD_GEN, 0.0, W1, vb);
+    gemm(N_, N_, vb, n, vb, 1.0, Tb, ldt, LD_UPPER, W1, vb, LD_GEN, 0.0, W2, vb);
+    gemm(N_, N_, ml, n, vb, -1.0, V, ldv, LD_VUNIT, W2, vb, LD_GEN, 1.0, C, ldc);
+    ld_ws_release(mark);
+}
+
+/* LAPACK dgeqrf on the GPU: the recursive panel's V and R, tau = diag of its
+ * T (dgeqr3 uses dlarfg, so these are dgeqrf's V, R and tau) */
+static void geqrf_tau(int ml, int n, double *A, int lda, double *tau) {
+    size_t mark = ld_ws_mark();
+    double *Tf = wsd((size_t)n * n);
+    dv_dgeqr3(ml, n, A, lda, Tf, n);
+    ld_T2tau(n, 0, n + 1, Tf, n, tau);
+    ld_ws_release(mark);
+}
+
+/* LAPACK dormqr('L', trans) with k reflectors given by V and tau: one block
+ * reflector with T = dlarft(V, tau) */
+static void orm_tau(char trans, int ml, int n, int k, const double *V, int ldv, const double *tau,
+                    double *C, int ldc) {
+    if (k <= 0 || n <= 0) return;
+    size_t mark = ld_ws_mark();
+    double *Tf = wsd((size_t)k * k);
+    ld_larft(ml, k, V, ldv, tau, Tf, k);
+    if (trans == 'T') apply_block_qt(ml, n, k, V, ldv, Tf, k, C, ldc);
+    else apply_block_q(ml, n, k, V, ldv, Tf, k, C, ldc);
+    ld_ws_release(mark);
+}
+
+/* LAPACK dorgqr: the first n columns of H_0 ... H_{n-1} */
+static void org_tau(int ml, int n, const double *V, int ldv, const double *tau, double *Q, int ldq) {
+    size_t mark = ld_ws_mark();
+    double *Tf = wsd((size_t)n * n);
+    ld_larft(ml, n, V, ldv, tau, Tf, n);
+    panel_q(ml, n, V, ldv, Tf, n, Q, ldq);
+    ld_ws_release(mark);
+}
+
+/* lila-c/lila_dgeqr2_v03_3.c:3-31 (its T V1^T product is never read) and
+ * lila-c/lila_dgeqr2_v03_l.c:3-25 (dgeqrf + dlarft: the same V, R, T) */
+int dv_dgeqr2_v03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt) {
+    return dv_dgeqr3(m - i, n, A + i + (size_t)i * lda, lda, T + i % mt + (size_t)i * ldt, ldt);
+}
+
+/* lila-c/lila_dgeqr2_v03.c:3-19 */
+int dv_dgeqr2_v03(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt) {
+    if (p[2] == 0 || p[2] == 1) return dv_dgeqr2_v03_3(m, n, i, mt, A, lda, T, ldt);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_mt_hh.c:3-35 */
+int dv_dgeqrf_v03_mt_hh(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                        int ldt) {
+    int vb = mt - i % mt;
+    if (vb > n) vb = n;
+    dv_dgeqr2_v03(p, m, vb, i, mt, A, lda, T, ldt);
+    int j = i + vb, l = vb;
+    vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    while (vb != 0) {
+        dv_dormqrf_w03(m, vb, l, i, j, mt, A, lda, T, ldt);
+        dv_dgeqr2_v03(p, m, vb, j, mt, A, lda, T, ldt);
+        j += vb;
+        l += vb;
+        vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    }
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_mt_l.c:3-20: dgeqrf, then lila_dlarft_w03 */
+int dv_dgeqrf_v03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt) {
+    size_t mark = ld_ws_mark();
+    double *tau = wsd((size_t)n);
+    geqrf_tau(m - i, n, A + i + (size_t)i * lda, lda, tau);
+    dv_dlarft_w03(m, n, i, mt, A, lda, T, ldt, tau);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_mt.c:3-19 */
+int dv_dgeqrf_v03_mt(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                     int ldt) {
+    if (p[1] == 0) return dv_dgeqrf_v03_mt_l(m, n, i, mt, A, lda, T, ldt);
+    if (p[1] == 1) return dv_dgeqrf_v03_mt_hh(p, m, n, i, mt, A, lda, T, ldt);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_recursive.c:3-29 (leaf when n < nx) */
+int dv_dgeqrf_v03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                            int ldt) {
+    if (n < p[3]) return dv_dgeqrf_v03_mt(p, m, n, i, mt, A, lda, T, ldt);
+    int n1 = n / 2, n2 = n - n1;
+    dv_dgeqrf_v03_recursive(p, m, n1, i, mt, A, lda, T, ldt);
+    dv_dormqrf_w03(m, n2, n1, i, i + n1, mt, A, lda, T, ldt);
+    dv_dgeqrf_v03_recursive(p, m, n2, i + n1, mt, A, lda, T, ldt);
+    dv_dlarft_connect_w03(m, n2, i + n1, i, mt, A, lda, T, ldt);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_levelx.c:3-52 */
+int dv_dgeqrf_v03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
+                         int mt, double *A, int lda, double *T, int ldt) {
+    int nb = nb_lvl[i_lvl];
+    int vb = nb > n ? n : nb;
+    int k = 0, j = i;
+    if (i_lvl == n_lvl - 1) dv_dgeqrf_v03_mt(p, m, vb, i, mt, A, lda, T, ldt);
+    else dv_dgeqrf_v03_levelx(p, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda, T, ldt);
+    k += vb;
+    j += vb;
+    while (j < i + n) {
+        vb = (j + nb > i + n) ? (i + n - j) : nb;
+        dv_dormqrf_w03(m, vb, k, i, j, mt, A, lda, T, ldt);
+        if (i_lvl == n_lvl - 1) dv_dgeqrf_v03_mt(p, m, vb, j, mt, A, lda, T, ldt);
+        else dv_dgeqrf_v03_levelx(p, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda, T, ldt);
+        dv_dlarft_connect_w03(m, vb, j, i, mt, A, lda, T, ldt);
+        k += vb;
+        j += vb;
+    }
+    return 0;
+}
+
+/* panel Q from the stored (V, T): lila-c/lila_dgeqr2_q03_3.c:3-32 */
+int dv_dgeqr2_q03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                    int ldq) {
+    panel_q(m - i, n, A + i + (size_t)i * lda, lda, T + i % mt + (size_t)i * ldt, ldt,
+            Q + i + (size_t)i * ldq, ldq);
+    return 0;
+}
+
+/* panel Q through dorgqr with tau = the T block's diagonal (its off-diagonal
+ * part is not read): lila-c/lila_dgeqr2_q03_l.c:3-25 */
+int dv_dgeqr2_q03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                    int ldq) {
+    size_t mark = ld_ws_mark();
+    double *tau = wsd((size_t)n);
+    ld_T2tau(n, i, mt, T, ldt, tau);
+    org_tau(m - i, n, A + i + (size_t)i * lda, lda, tau, Q + i + (size_t)i * ldq, ldq);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* lila-c/lila_dgeqr2_q03.c:3-19 */
+int dv_dgeqr2_q03(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                  double *Q, int ldq) {
+    if (p[2] == 0) return dv_dgeqr2_q03_l(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    if (p[2] == 1) return dv_dgeqr2_q03_3(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_mt_hh.c:3-32 */
+int dv_dgeqrf_q03_mt_hh(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                        int ldt, double *Q, int ldq) {
+    int vb = mt - i % mt;
+    if (vb > n) vb = n;
+    dv_dgeqr2_q03(p, m, vb, i, mt, A, lda, T, ldt, Q, ldq);
+    int j = i + vb, l = vb;
+    vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    while (vb != 0) {
+        dv_dgeqr2_q03(p, m, vb, j, mt, A, lda, T, ldt, Q, ldq);
+        dv_dormqrbz_w03(m, vb, l, i, j, mt, A, lda, Q, ldq, T, ldt);
+        j += vb;
+        l += vb;
+        vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    }
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_mt_l.c:3-23: dT2tau over the n columns, dorgqr */
+int dv_dgeqrf_q03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                       int ldq) {
+    size_t mark = ld_ws_mark();
+    double *tau = wsd((size_t)n);
+    ld_T2tau(n, i, mt, T, ldt, tau);
+    org_tau(m - i, n, A + i + (size_t)i * lda, lda, tau, Q + i + (size_t)i * ldq, ldq);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_mt.c:3-19 */
+int dv_dgeqrf_q03_mt(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                     double *Q, int ldq) {
+    if (p[1] == 0) return dv_dgeqrf_q03_mt_l(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    if (p[1] == 1) return dv_dgeqrf_q03_mt_hh(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_recursive.c:3-25 (leaf when n < nx) */
+int dv_dgeqrf_q03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                            int ldt, double *Q, int ldq) {
+    if (n < p[3]) return dv_dgeqrf_q03_mt(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    int n1 = n / 2, n2 = n - n1;
+    dv_dgeqrf_q03_recursive(p, m, n1, i, mt, A, lda, T, ldt, Q, ldq);
+    dv_dgeqrf_q03_recursive(p, m, n2, i + n1, mt, A, lda, T, ldt, Q, ldq);
+    dv_dormqrbz_w03(m, n2, n1, i, i + n1, mt, A, lda, Q, ldq, T, ldt);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_levelx.c:3-48 */
+int dv_dgeqrf_q03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
+                         int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq) {
+    int nb = nb_lvl[i_lvl];
+    int vb = nb > n ? n : nb;
+    int k = 0, j = i;
+    if (i_lvl == n_lvl - 1) dv_dgeqrf_q03_mt(p, m, vb, i, mt, A, lda, T, ldt, Q, ldq);
+    else dv_dgeqrf_q03_levelx(p, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda, T, ldt, Q, ldq);
+    k += vb;
+    j += vb;
+    while (j < i + n) {
+        vb = (j + nb > i + n) ? (i + n - j) : nb;
+        if (i_lvl == n_lvl - 1) dv_dgeqrf_q03_mt(p, m, vb, j, mt, A, lda, T, ldt, Q, ldq);
+        else dv_dgeqrf_q03_levelx(p, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda, T, ldt, Q, ldq);
+        dv_dormqrbz_w03(m, vb, k, i, j, mt, A, lda, Q, ldq, T, ldt);
+        k += vb;
+        j += vb;
+    }
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_w03_appendcols.c:3-14 */
+int dv_dgeqrf_w03_appendcols(const int *p, int m, int k, int n, int mt, double *A, int lda, double *Q,
+                             int ldq, double *T, int ldt) {
+    if (k != 0) dv_dormqrf_w03(m, n, k, 0, k, mt, A, lda, T, ldt);
+    dv_dgeqrf_w03_recursive(p, m, n, k, mt, A, lda, T, ldt, Q, ldq);
+    if (k != 0) dv_dlarft_connect_w03(m, n, k, 0, mt, A, lda, T, ldt);
+    if (k != 0) dv_dormqrbz_w03(m, n, k, 0, k, mt, A, lda, Q, ldq, T, ldt);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_LAPACK_appendcols.c:3-30, tau in T (k + n entries);
+ * A's new columns use lda (the reference passes ldq, :20) */
+int dv_dgeqrf_LAPACK_appendcols(int m, int k, int n, double *A, int lda, double *Q, int ldq,
+                                double *tau) {
+    int ml = m - k;
+    if (k != 0) orm_tau('T', m, n, k, A, lda, tau, A + (size_t)k * lda, lda);
+    geqrf_tau(ml, n, A + k + (size_t)k * lda, lda, tau + k);
+    org_tau(ml, n, A + k + (size_t)k * lda, lda, tau + k, Q + k + (size_t)k * ldq, ldq);
+    if (k != 0) {
+        ld_laset('A', k, n, 0.0, 0.0, Q + (size_t)k * ldq, ldq);
+        orm_tau('N', m, n, k, A, lda, tau, Q + (size_t)k * ldq, ldq);
+    }
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_ker_recursive.c:3-44, tau of the columns from i at
+ * T + i; A's right half uses lda (the reference passes ldq, :31) */
+int dv_dgeqrf_ker_recursive(const int *p, int m, int n, int i, double *A, int lda, double *T, double *Q,
+                            int ldq) {
+    double *tau = T + i;
+    double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
+    int ml = m - i;
+    if (n <= p[3]) {
+        geqrf_tau(ml, n, Aii, lda, tau);
+        org_tau(ml, n, Aii, lda, tau, Qii, ldq);
+        return 0;
+    }
+    int n1 = n / 2, n2 = n - n1;
+    dv_dgeqrf_ker_recursive(p, m, n1, i, A, lda, T, Q, ldq);
+    orm_tau('T', ml, n2, n1, Aii, lda, tau, Aii + (size_t)n1 * lda, lda);
+    dv_dgeqrf_ker_recursive(p, m, n2, i + n1, A, lda, T, Q, ldq);
+    ld_laset('A', n1, n2, 0.0, 0.0, Qii + (size_t)n1 * ldq, ldq);
+    orm_tau('N', ml, n2, n1, Aii, lda, tau, Qii + (size_t)n1 * ldq, ldq);
+    return 0;
+}
+
+/* tau_p = 2 / (1 + ||v_p(1:)||^2), then dormqr: lila-c/lila_dormqrf_w00.c:3-29
+ * (C = A(i:m, j:j+n)) and lila-c/lila_dormqrf_z00.c:3-31 (C = B(i:m, j:j+n)) */
+int dv_dormqrf_tau_from_v(int m, int n, int k, int i, const double *A, int lda, double *C, int ldc) {
+    if (k <= 0 || n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *tau = wsd((size_t)k);
+    ld_V2tau(m, k, i, A, lda, tau);
+    orm_tau('T', m - i, n, k, A + i + (size_t)i * lda, lda, tau, C, ldc);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* one block with the full-layout T at T(i, i): lila-c/lila_dormqrf_z02.c:3-42 */
+int dv_dormqrf_z02(int m, int n, int k, int i, int j, double *A, int lda, double *T, int ldt, double *B,
+                   int ldb) {
+    apply_block_qt(m - i, n, k, A + i + (size_t)i * lda, lda, T + i + (size_t)i * ldt, ldt,
+                   B + i + (size_t)j * ldb, ldb);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_qr2.c:3-45 (lila_param[4]: 0 VRTQ, 1 VRT, 2 Q) */
+int dv_dgeqrf_qr2(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                  double *Q, int ldq) {
+    if (p[4] == 0) return dv_dgeqr2_w03(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    if (p[4] == 1) return dv_dgeqr2_v03(p, m, n, i, mt, A, lda, T, ldt);
+    if (p[4] == 2) return dv_dgeqr2_q03(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_qr2_recursive.c:3-24 */
+int dv_dgeqrf_qr2_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                            int ldt, double *Q, int ldq) {
+    if (n <= p[3]) return dv_dgeqrf_qr2(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    int n1 = n / 2, n2 = n - n1;
+    return dv_dgeqrf_qr2_recursive2(p, m, n, i, mt, n1, n2, A, lda, T, ldt, Q, ldq);
+}
+
+/* lila-c/lila_dgeqrf_qr2_recursive2.c:3-34 */
+int dv_dgeqrf_qr2_recursive2(const int *p, int m, int n, int i, int mt, int nb1, int nb2, double *A,
+                             int lda, double *T, int ldt, double *Q, int ldq) {
+    (void)n;
+    if (p[4] == 0 || p[4] == 1) {
+        dv_dgeqrf_qr2_recursive(p, m, nb1, i, mt, A, lda, T, ldt, Q, ldq);
+        dv_dormqrf_w03(m, nb2, nb1, i, i + nb1, mt, A, lda, T, ldt);
+        dv_dgeqrf_qr2_recursive(p, m, nb2, i + nb1, mt, A, lda, T, ldt, Q, ldq);
+        dv_dlarft_connect_w03(m, nb2, i + nb1, i, mt, A, lda, T, ldt);
+        if (p[4] == 0) dv_dormqrbz_w03(m, nb2, nb1, i, i + nb1, mt, A, lda, Q, ldq, T, ldt);
+    } else if (p[4] == 2) {
+        dv_dgeqrf_qr2_recursive(p, m, nb1, i, mt, A, lda, T, ldt, Q, ldq);
+        dv_dgeqrf_qr2_recursive(p, m, nb2, i + nb1, mt, A, lda, T, ldt, Q, ldq);
+        dv_dormqrbz_w03(m, nb2, nb1, i, i + nb1, mt, A, lda, Q, ldq, T, ldt);
+    }
+    return 0;
+}
+
 /* ---- workspace queries: the reference's closed forms (lila-c/lila_query_*.c) */
 
 int dq_dormqrf_w03(int m, int n, int k, int i, int mt) {

@@ -40,6 +40,49 @@ int dv_dgeqrf_w03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, 
 int dv_dgetrf_b(int m, int n, int i, int mt, double *A, int lda, double *Q, int ldq, double *U);
 int dv_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb);
 
+int dv_dgeqr2_v03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt);
+int dv_dgeqr2_v03(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt);
+int dv_dgeqrf_v03_mt_hh(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                        int ldt);
+int dv_dgeqrf_v03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt);
+int dv_dgeqrf_v03_mt(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                     int ldt);
+int dv_dgeqrf_v03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                            int ldt);
+int dv_dgeqrf_v03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
+                         int mt, double *A, int lda, double *T, int ldt);
+int dv_dgeqr2_q03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                    int ldq);
+int dv_dgeqr2_q03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                    int ldq);
+int dv_dgeqr2_q03(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                  double *Q, int ldq);
+int dv_dgeqrf_q03_mt_hh(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                        int ldt, double *Q, int ldq);
+int dv_dgeqrf_q03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                       int ldq);
+int dv_dgeqrf_q03_mt(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                     double *Q, int ldq);
+int dv_dgeqrf_q03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                            int ldt, double *Q, int ldq);
+int dv_dgeqrf_q03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
+                         int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq);
+int dv_dgeqrf_w03_appendcols(const int *p, int m, int k, int n, int mt, double *A, int lda, double *Q,
+                             int ldq, double *T, int ldt);
+int dv_dgeqrf_LAPACK_appendcols(int m, int k, int n, double *A, int lda, double *Q, int ldq,
+                                double *tau);
+int dv_dgeqrf_ker_recursive(const int *p, int m, int n, int i, double *A, int lda, double *T, double *Q,
+                            int ldq);
+int dv_dormqrf_tau_from_v(int m, int n, int k, int i, const double *A, int lda, double *C, int ldc);
+int dv_dormqrf_z02(int m, int n, int k, int i, int j, double *A, int lda, double *T, int ldt, double *B,
+                   int ldb);
+int dv_dgeqrf_qr2(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                  double *Q, int ldq);
+int dv_dgeqrf_qr2_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
+                            int ldt, double *Q, int ldq);
+int dv_dgeqrf_qr2_recursive2(const int *p, int m, int n, int i, int mt, int nb1, int nb2, double *A,
+                             int lda, double *T, int ldt, double *Q, int ldq);
+
 int dq_dormqrf_w03(int m, int n, int k, int i, int mt);
 int dq_dormqrbz_w03(int m, int n, int k, int i, int mt);
 int dq_dgeqrf_w03_mt(const int *p, int m, int n, int i, int mt, int lwork);

@@ -668,7 +668,7 @@ __device__ __noinline__ void hh_fallback(const PanelArgs &a, double *slab, doubl
   hh_factor<false>(a, slab, tail, r0, nloc, nbar);
 }
 
-__global__ __launch_bounds__(PNT) void panel_cqr_kernel(PanelArgs a) {
+__global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) void panel_cqr_kernel(PanelArgs a) {
   extern __shared__ double smem[];
   const int nb = a.nb, RS = a.rs, G = a.G;
   const int g = blockIdx.x, tid = threadIdx.x, wave = tid >> 6;

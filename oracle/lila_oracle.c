@@ -917,6 +917,434 @@ int ora_lila_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl,
     return 0;
 }
 
+/* ======================================================================== */
+/* VRT-only (v03), Q-from-(V,T) (q03), append-columns, LAPACK-style and      */
+/* qr2 families of lila-c/lila.h:19-22, :74-90, :95-102                     */
+/* ======================================================================== */
+
+/* LAPACK DORM2R, side 'L': C <- Q^T C (trans 'T') or Q C ('N'),
+ * Q = H_0 ... H_{k-1}, H_p = I - tau_p v_p v_p^T, v_p = A(p:m, p), v_p(0) = 1 */
+static void o_dorm2r(char trans, int m, int n, int k, const double *A, int lda,
+                     const double *tau, double *C, int ldc) {
+    double *v = (double *)xmalloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+    for (int t = 0; t < k; t++) {
+        int p = (trans == 'T') ? t : k - 1 - t;
+        v[0] = 1.0;
+        for (int r = 1; r < m - p; r++) v[r] = IDX(A, p + r, p, lda);
+        o_dlarf_left(m - p, n, v, tau[p], &IDX(C, p, 0, ldc), ldc);
+    }
+    free(v);
+}
+
+/* dgeqr3 panel, T into the mt layout; lila-c/lila_dgeqr2_v03_3.c:3-31 (its
+ * trailing T V1^T product lands in work and is never read: not restated) */
+int ora_lila_dgeqr2_v03_3(int m, int n, int i, int mt, double *A, int lda,
+                          double *T, int ldt, double *work, int lwork) {
+    (void)work; (void)lwork;
+    ora_dgeqr3(m - i, n, &IDX(A, i, i, lda), lda, &IDX(T, i % mt, i, ldt), ldt);
+    return 0;
+}
+
+/* LAPACK panel: dgeqrf + dlarft; lila-c/lila_dgeqr2_v03_l.c:3-25 */
+int ora_lila_dgeqr2_v03_l(int m, int n, int i, int mt, double *A, int lda,
+                          double *T, int ldt, double *work, int lwork) {
+    (void)work; (void)lwork;
+    double *tau = (double *)xmalloc((size_t)n * sizeof(double));
+    double *Aii = &IDX(A, i, i, lda);
+    ora_dgeqr2(m - i, n, Aii, lda, tau);
+    ora_dlarft_fc(m - i, n, Aii, lda, tau, &IDX(T, i % mt, i, ldt), ldt);
+    free(tau);
+    return 0;
+}
+
+/* lila-c/lila_dgeqr2_v03.c:3-19 (panel 2 is commented out there: no-op) */
+int ora_lila_dgeqr2_v03(int *lila_param, int m, int n, int i, int mt, double *A,
+                        int lda, double *T, int ldt, double *work, int lwork) {
+    if (lila_param[2] == 0) return ora_lila_dgeqr2_v03_l(m, n, i, mt, A, lda, T, ldt, work, lwork);
+    if (lila_param[2] == 1) return ora_lila_dgeqr2_v03_3(m, n, i, mt, A, lda, T, ldt, work, lwork);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_mt_hh.c:3-35 */
+int ora_lila_dgeqrf_v03_mt_hh(int *lila_param, int m, int n, int i, int mt,
+                              double *A, int lda, double *T, int ldt,
+                              double *work, int lwork) {
+    int vb = mt - i % mt;
+    if (vb > n) vb = n;
+    ora_lila_dgeqr2_v03(lila_param, m, vb, i, mt, A, lda, T, ldt, work, lwork);
+    int j = i + vb, l = vb;
+    vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    while (vb != 0) {
+        ora_lila_dormqrf_w03(m, vb, l, i, j, mt, A, lda, T, ldt, work, lwork);
+        ora_lila_dgeqr2_v03(lila_param, m, vb, j, mt, A, lda, T, ldt, work, lwork);
+        j += vb;
+        l += vb;
+        vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    }
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_mt_l.c:3-20 (there tau sits inside dgeqrf's own
+ * workspace; here it has storage of its own) */
+int ora_lila_dgeqrf_v03_mt_l(int m, int n, int i, int mt, double *A, int lda,
+                             double *T, int ldt, double *work, int lwork) {
+    (void)work; (void)lwork;
+    double *tau = (double *)xmalloc((size_t)n * sizeof(double));
+    ora_dgeqr2(m - i, n, &IDX(A, i, i, lda), lda, tau);
+    ora_lila_dlarft_w03(m, n, i, mt, A, lda, T, ldt, tau);
+    free(tau);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_mt.c:3-19 */
+int ora_lila_dgeqrf_v03_mt(int *lila_param, int m, int n, int i, int mt,
+                           double *A, int lda, double *T, int ldt,
+                           double *work, int lwork) {
+    if (lila_param[1] == 0) return ora_lila_dgeqrf_v03_mt_l(m, n, i, mt, A, lda, T, ldt, work, lwork);
+    if (lila_param[1] == 1)
+        return ora_lila_dgeqrf_v03_mt_hh(lila_param, m, n, i, mt, A, lda, T, ldt, work, lwork);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_recursive.c:3-29 (leaf when n < nx) */
+int ora_lila_dgeqrf_v03_recursive(int *lila_param, int m, int n, int i, int mt,
+                                  double *A, int lda, double *T, int ldt,
+                                  double *work, int lwork) {
+    if (n < lila_param[3])
+        return ora_lila_dgeqrf_v03_mt(lila_param, m, n, i, mt, A, lda, T, ldt, work, lwork);
+    int n1 = n / 2, n2 = n - n1;
+    ora_lila_dgeqrf_v03_recursive(lila_param, m, n1, i, mt, A, lda, T, ldt, work, lwork);
+    ora_lila_dormqrf_w03(m, n2, n1, i, i + n1, mt, A, lda, T, ldt, work, lwork);
+    ora_lila_dgeqrf_v03_recursive(lila_param, m, n2, i + n1, mt, A, lda, T, ldt, work, lwork);
+    ora_lila_dlarft_connect_w03(m, n2, i + n1, i, mt, A, lda, T, ldt);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_v03_levelx.c:3-52 */
+int ora_lila_dgeqrf_v03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl,
+                               int m, int n, int i, int mt, double *A, int lda,
+                               double *T, int ldt, double *work, int lwork) {
+    int nb = nb_lvl[i_lvl];
+    int vb = nb > n ? n : nb;
+    int k = 0, j = i;
+    if (i_lvl == n_lvl - 1)
+        ora_lila_dgeqrf_v03_mt(lila_param, m, vb, i, mt, A, lda, T, ldt, work, lwork);
+    else
+        ora_lila_dgeqrf_v03_levelx(lila_param, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda, T,
+                                   ldt, work, lwork);
+    k += vb;
+    j += vb;
+    while (j < i + n) {
+        vb = (j + nb > i + n) ? (i + n - j) : nb;
+        ora_lila_dormqrf_w03(m, vb, k, i, j, mt, A, lda, T, ldt, work, lwork);
+        if (i_lvl == n_lvl - 1)
+            ora_lila_dgeqrf_v03_mt(lila_param, m, vb, j, mt, A, lda, T, ldt, work, lwork);
+        else
+            ora_lila_dgeqrf_v03_levelx(lila_param, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda,
+                                       T, ldt, work, lwork);
+        ora_lila_dlarft_connect_w03(m, vb, j, i, mt, A, lda, T, ldt);
+        k += vb;
+        j += vb;
+    }
+    return 0;
+}
+
+/* panel Q from (V, T): Q = [I;0] - V (T V1^T); lila-c/lila_dgeqr2_q03_3.c:3-32 */
+int ora_lila_dgeqr2_q03_3(int m, int n, int i, int mt, double *A, int lda,
+                          double *T, int ldt, double *Q, int ldq,
+                          double *work, int lwork) {
+    (void)work; (void)lwork;
+    int ml = m - i;
+    double *Aii = &IDX(A, i, i, lda), *Qii = &IDX(Q, i, i, ldq);
+    double *Tki = &IDX(T, i % mt, i, ldt);
+    double *W = (double *)xmalloc((size_t)n * n * sizeof(double));
+    ora_dlacpy('A', ml, n, Aii, lda, Qii, ldq);
+    ora_dlaset('U', n, n, 0.0, 1.0, Qii, ldq);
+    ora_dlaset('A', n, n, 0.0, 0.0, W, n);
+    ora_dlacpy('U', n, n, Tki, ldt, W, n);
+    ora_dtrmm('R', 'L', 'T', 'U', n, n, 1.0, Aii, lda, W, n);
+    ora_dtrmm('R', 'U', 'N', 'N', ml, n, -1.0, W, n, Qii, ldq);
+    for (int c = 0; c < n; c++) IDX(Qii, c, c, ldq) += 1.0;
+    free(W);
+    return 0;
+}
+
+/* panel Q through dorgqr with tau = diag of the T block;
+ * lila-c/lila_dgeqr2_q03_l.c:3-25 */
+int ora_lila_dgeqr2_q03_l(int m, int n, int i, int mt, double *A, int lda,
+                          double *T, int ldt, double *Q, int ldq,
+                          double *work, int lwork) {
+    (void)work; (void)lwork;
+    int ml = m - i;
+    double *tau = (double *)xmalloc((size_t)n * sizeof(double));
+    const double *Tki = &IDX(T, i % mt, i, ldt);
+    for (int c = 0; c < n; c++) tau[c] = IDX(Tki, c, c, ldt);
+    ora_dlacpy('A', ml, n, &IDX(A, i, i, lda), lda, &IDX(Q, i, i, ldq), ldq);
+    ora_dorg2r(ml, n, n, &IDX(Q, i, i, ldq), ldq, tau);
+    free(tau);
+    return 0;
+}
+
+/* lila-c/lila_dgeqr2_q03.c:3-19 */
+int ora_lila_dgeqr2_q03(int *lila_param, int m, int n, int i, int mt, double *A,
+                        int lda, double *T, int ldt, double *Q, int ldq,
+                        double *work, int lwork) {
+    if (lila_param[2] == 0)
+        return ora_lila_dgeqr2_q03_l(m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    if (lila_param[2] == 1)
+        return ora_lila_dgeqr2_q03_3(m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_mt_hh.c:3-32 */
+int ora_lila_dgeqrf_q03_mt_hh(int *lila_param, int m, int n, int i, int mt,
+                              double *A, int lda, double *T, int ldt,
+                              double *Q, int ldq, double *work, int lwork) {
+    int vb = mt - i % mt;
+    if (vb > n) vb = n;
+    ora_lila_dgeqr2_q03(lila_param, m, vb, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    int j = i + vb, l = vb;
+    vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    while (vb != 0) {
+        ora_lila_dgeqr2_q03(lila_param, m, vb, j, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+        ora_lila_dormqrbz_w03(m, vb, l, i, j, mt, A, lda, Q, ldq, T, ldt, work, lwork);
+        j += vb;
+        l += vb;
+        vb = (j + mt >= i + n) ? (n - (j - i)) : mt;
+    }
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_mt_l.c:3-23 */
+int ora_lila_dgeqrf_q03_mt_l(int m, int n, int i, int mt, double *A, int lda,
+                             double *T, int ldt, double *Q, int ldq,
+                             double *work, int lwork) {
+    (void)work; (void)lwork;
+    int ml = m - i;
+    double *tau = (double *)xmalloc((size_t)n * sizeof(double));
+    ora_lila_dT2tau_w03(m, n, i, mt, T, ldt, tau);
+    ora_dlacpy('A', ml, n, &IDX(A, i, i, lda), lda, &IDX(Q, i, i, ldq), ldq);
+    ora_dorg2r(ml, n, n, &IDX(Q, i, i, ldq), ldq, tau);
+    free(tau);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_mt.c:3-19 */
+int ora_lila_dgeqrf_q03_mt(int *lila_param, int m, int n, int i, int mt,
+                           double *A, int lda, double *T, int ldt,
+                           double *Q, int ldq, double *work, int lwork) {
+    if (lila_param[1] == 0)
+        return ora_lila_dgeqrf_q03_mt_l(m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    if (lila_param[1] == 1)
+        return ora_lila_dgeqrf_q03_mt_hh(lila_param, m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_recursive.c:3-25 (leaf when n < nx) */
+int ora_lila_dgeqrf_q03_recursive(int *lila_param, int m, int n, int i, int mt,
+                                  double *A, int lda, double *T, int ldt,
+                                  double *Q, int ldq, double *work, int lwork) {
+    if (n < lila_param[3])
+        return ora_lila_dgeqrf_q03_mt(lila_param, m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    int n1 = n / 2, n2 = n - n1;
+    ora_lila_dgeqrf_q03_recursive(lila_param, m, n1, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    ora_lila_dgeqrf_q03_recursive(lila_param, m, n2, i + n1, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    ora_lila_dormqrbz_w03(m, n2, n1, i, i + n1, mt, A, lda, Q, ldq, T, ldt, work, lwork);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_q03_levelx.c:3-48 */
+int ora_lila_dgeqrf_q03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl,
+                               int m, int n, int i, int mt, double *A, int lda,
+                               double *T, int ldt, double *Q, int ldq,
+                               double *work, int lwork) {
+    int nb = nb_lvl[i_lvl];
+    int vb = nb > n ? n : nb;
+    int k = 0, j = i;
+    if (i_lvl == n_lvl - 1)
+        ora_lila_dgeqrf_q03_mt(lila_param, m, vb, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    else
+        ora_lila_dgeqrf_q03_levelx(lila_param, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda, T,
+                                   ldt, Q, ldq, work, lwork);
+    k += vb;
+    j += vb;
+    while (j < i + n) {
+        vb = (j + nb > i + n) ? (i + n - j) : nb;
+        if (i_lvl == n_lvl - 1)
+            ora_lila_dgeqrf_q03_mt(lila_param, m, vb, j, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+        else
+            ora_lila_dgeqrf_q03_levelx(lila_param, n_lvl, i_lvl + 1, nb_lvl, m, vb, j, mt, A, lda,
+                                       T, ldt, Q, ldq, work, lwork);
+        ora_lila_dormqrbz_w03(m, vb, k, i, j, mt, A, lda, Q, ldq, T, ldt, work, lwork);
+        k += vb;
+        j += vb;
+    }
+    return 0;
+}
+
+/* QR of k existing columns extended by n new ones (Block-Arnoldi use):
+ * lila-c/lila_dgeqrf_w03_appendcols.c:3-14 */
+int ora_lila_dgeqrf_w03_appendcols(int *lila_param, int m, int k, int n, int mt,
+                                   double *A, int lda, double *Q, int ldq,
+                                   double *T, int ldt, double *work, int lwork) {
+    if (k != 0) ora_lila_dormqrf_w03(m, n, k, 0, k, mt, A, lda, T, ldt, work, lwork);
+    ora_lila_dgeqrf_w03_recursive(lila_param, m, n, k, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    if (k != 0) ora_lila_dlarft_connect_w03(m, n, k, 0, mt, A, lda, T, ldt);
+    if (k != 0) ora_lila_dormqrbz_w03(m, n, k, 0, k, mt, A, lda, Q, ldq, T, ldt, work, lwork);
+    return 0;
+}
+
+/* the same through LAPACK conventions, tau stored in T (a vector of k + n):
+ * lila-c/lila_dgeqrf_LAPACK_appendcols.c:3-30.  The reference passes ldq as
+ * the leading dimension of A's new columns in its first dormqr (:20); the
+ * two agree when lda == ldq, and lda is used here. */
+int ora_lila_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A,
+                                      int lda, double *Q, int ldq, double *T,
+                                      int ldt, double *work, int lwork) {
+    (void)mt; (void)ldt; (void)work; (void)lwork;
+    double *tau = T;
+    int ml = m - k;
+    double *Aji = &IDX(A, 0, k, lda), *Qji = &IDX(Q, 0, k, ldq);
+    if (k != 0) o_dorm2r('T', m, n, k, A, lda, tau, Aji, lda);
+    ora_dgeqr2(ml, n, &IDX(A, k, k, lda), lda, tau + k);
+    ora_dlacpy('A', ml, n, &IDX(A, k, k, lda), lda, &IDX(Q, k, k, ldq), ldq);
+    ora_dorg2r(ml, n, n, &IDX(Q, k, k, ldq), ldq, tau + k);
+    if (k != 0) ora_dlaset('A', k, n, 0.0, 0.0, Qji, ldq);
+    if (k != 0) o_dorm2r('N', m, n, k, A, lda, tau, Qji, ldq);
+    return 0;
+}
+
+/* recursive QR on LAPACK kernels, tau stored in T (T + i holds the tau of
+ * the columns from i): lila-c/lila_dgeqrf_ker_recursive.c:3-44.  The
+ * reference's dormqr on A's right half passes ldq as its leading dimension
+ * (:31); lda is used here (the same when lda == ldq). */
+int ora_lila_dgeqrf_ker_recursive(int *lila_param, int m, int n, int i, int mt,
+                                  double *A, int lda, double *T, int ldt,
+                                  double *Q, int ldq, double *work, int lwork) {
+    (void)ldt;
+    double *tau = T + i;
+    double *Aii = &IDX(A, i, i, lda), *Qii = &IDX(Q, i, i, ldq);
+    int ml = m - i;
+    if (n <= lila_param[3]) {
+        ora_dgeqr2(ml, n, Aii, lda, tau);
+        ora_dlacpy('A', ml, n, Aii, lda, Qii, ldq);
+        ora_dorg2r(ml, n, n, Qii, ldq, tau);
+        return 0;
+    }
+    int n1 = n / 2, n2 = n - n1;
+    ora_lila_dgeqrf_ker_recursive(lila_param, m, n1, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    o_dorm2r('T', ml, n2, n1, Aii, lda, tau, Aii + (size_t)n1 * lda, lda);
+    ora_lila_dgeqrf_ker_recursive(lila_param, m, n2, i + n1, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    ora_dlaset('A', n1, n2, 0.0, 0.0, Qii + (size_t)n1 * ldq, ldq);
+    o_dorm2r('N', ml, n2, n1, Aii, lda, tau, Qii + (size_t)n1 * ldq, ldq);
+    return 0;
+}
+
+/* tau_p = 2 / (1 + ||v_p(1:)||^2) from the stored reflectors, then dormqr:
+ * lila-c/lila_dormqrf_w00.c:3-29 (C = A(i:m, j:j+n)) and
+ * lila-c/lila_dormqrf_z00.c:3-31 (C = B(i:m, j:j+n)) */
+static void o_dormqrf_tau_from_v(int m, int n, int k, int i, const double *A, int lda,
+                                 double *C, int ldc) {
+    int ml = m - i;
+    double *tau = (double *)xmalloc((size_t)(k > 0 ? k : 1) * sizeof(double));
+    for (int p = 0; p < k; p++) {
+        double s = 1.0;
+        for (int r = i + p + 1; r < m; r++) s += IDX(A, r, i + p, lda) * IDX(A, r, i + p, lda);
+        tau[p] = 2.0 / s;
+    }
+    o_dorm2r('T', ml, n, k, &IDX(A, i, i, lda), lda, tau, C, ldc);
+    free(tau);
+}
+
+int ora_lila_dormqrf_w00(int m, int n, int k, int i, int j, int mt, double *A, int lda,
+                         double *T, int ldt, double *work, int lwork) {
+    (void)mt; (void)T; (void)ldt; (void)work; (void)lwork;
+    o_dormqrf_tau_from_v(m, n, k, i, A, lda, &IDX(A, i, j, lda), lda);
+    return 0;
+}
+
+int ora_lila_dormqrf_z00(int m, int n, int k, int i, int j, int mt, double *A, int lda,
+                         double *B, int ldb, double *T, int ldt, double *work, int lwork) {
+    (void)mt; (void)T; (void)ldt; (void)work; (void)lwork;
+    o_dormqrf_tau_from_v(m, n, k, i, A, lda, &IDX(B, i, j, ldb), ldb);
+    return 0;
+}
+
+/* one block of k reflectors with the full-layout T at T(i, i):
+ * lila-c/lila_dormqrf_z02.c:3-42 */
+int ora_lila_dormqrf_z02(int m, int n, int k, int i, int j, int mt, double *A, int lda,
+                         double *T, int ldt, double *B, int ldb, double *work, int lwork) {
+    (void)mt; (void)work; (void)lwork;
+    const double *Aii = &IDX(A, i, i, lda), *Tii = &IDX(T, i, i, ldt);
+    double *Bij = &IDX(B, i, j, ldb);
+    double *W = (double *)xmalloc((size_t)(k > 0 ? k : 1) * n * sizeof(double));
+    for (int c = 0; c < n; c++)
+        for (int r = 0; r < k; r++) IDX(W, r, c, k) = IDX(Bij, r, c, ldb);
+    ora_dtrmm('L', 'L', 'T', 'U', k, n, 1.0, Aii, lda, W, k);
+    ora_dgemm('T', 'N', k, n, m - k - i, 1.0, Aii + k, lda, Bij + k, ldb, 1.0, W, k);
+    ora_dtrmm('L', 'U', 'T', 'N', k, n, 1.0, Tii, ldt, W, k);
+    ora_dgemm('N', 'N', m - k - i, n, k, -1.0, Aii + k, lda, W, k, 1.0, Bij + k, ldb);
+    ora_dtrmm('L', 'L', 'N', 'U', k, n, 1.0, Aii, lda, W, k);
+    for (int c = 0; c < n; c++)
+        for (int r = 0; r < k; r++) IDX(Bij, r, c, ldb) -= IDX(W, r, c, k);
+    free(W);
+    return 0;
+}
+
+/* panel by lila_param[4] (0 VRTQ, 1 VRT, 2 Q) and lila_param[2];
+ * lila-c/lila_dgeqrf_qr2.c:3-45 (the hr forms of 1 and 2 are commented out
+ * there: no-ops) */
+int ora_lila_dgeqrf_qr2(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                        double *T, int ldt, double *Q, int ldq, double *work, int lwork) {
+    int panel = lila_param[2];
+    if (lila_param[4] == 0) {
+        if (panel == 0) return ora_lila_dgeqr2_w03_l(m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+        if (panel == 1) return ora_lila_dgeqr2_w03_3(m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+        if (panel == 2) return ora_lila_dgeqr2_w03_hr(m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    } else if (lila_param[4] == 1) {
+        if (panel == 0) return ora_lila_dgeqr2_v03_l(m, n, i, mt, A, lda, T, ldt, work, lwork);
+        if (panel == 1) return ora_lila_dgeqr2_v03_3(m, n, i, mt, A, lda, T, ldt, work, lwork);
+    } else if (lila_param[4] == 2) {
+        if (panel == 0) return ora_lila_dgeqr2_q03_l(m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+        if (panel == 1) return ora_lila_dgeqr2_q03_3(m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    }
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_qr2_recursive.c:3-24 */
+int ora_lila_dgeqrf_qr2_recursive(int *lila_param, int m, int n, int i, int mt, double *A,
+                                  int lda, double *T, int ldt, double *Q, int ldq,
+                                  double *work, int lwork) {
+    if (n <= lila_param[3])
+        return ora_lila_dgeqrf_qr2(lila_param, m, n, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+    int n1 = n / 2, n2 = n - n1;
+    return ora_lila_dgeqrf_qr2_recursive2(lila_param, m, n, i, mt, n1, n2, A, lda, T, ldt, Q, ldq,
+                                          work, lwork);
+}
+
+/* the three step lists of lila-c/lila_dgeqrf_qr2_recursive2.c:3-34 */
+int ora_lila_dgeqrf_qr2_recursive2(int *lila_param, int m, int n, int i, int mt, int nb1, int nb2,
+                                   double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                                   double *work, int lwork) {
+    (void)n;
+    int method = lila_param[4];
+    if (method == 0 || method == 1) {
+        ora_lila_dgeqrf_qr2_recursive(lila_param, m, nb1, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+        ora_lila_dormqrf_w03(m, nb2, nb1, i, i + nb1, mt, A, lda, T, ldt, work, lwork);
+        ora_lila_dgeqrf_qr2_recursive(lila_param, m, nb2, i + nb1, mt, A, lda, T, ldt, Q, ldq, work,
+                                      lwork);
+        ora_lila_dlarft_connect_w03(m, nb2, i + nb1, i, mt, A, lda, T, ldt);
+        if (method == 0)
+            ora_lila_dormqrbz_w03(m, nb2, nb1, i, i + nb1, mt, A, lda, Q, ldq, T, ldt, work, lwork);
+    } else if (method == 2) {
+        ora_lila_dgeqrf_qr2_recursive(lila_param, m, nb1, i, mt, A, lda, T, ldt, Q, ldq, work, lwork);
+        ora_lila_dgeqrf_qr2_recursive(lila_param, m, nb2, i + nb1, mt, A, lda, T, ldt, Q, ldq, work,
+                                      lwork);
+        ora_lila_dormqrbz_w03(m, nb2, nb1, i, i + nb1, mt, A, lda, Q, ldq, T, ldt, work, lwork);
+    }
+    return 0;
+}
+
 /* In-place sign-chosen unpivoted LU of A(i:m, i:i+n) (holding a copy of the
  * orthonormal Q): per column, if |1 - a_kk| < |1 + a_kk| negate row k of R
  * (held in work, n x n, ld n) and column k of Q(i:m, i:i+n), otherwise negate

@@ -115,6 +115,62 @@ int ora_lila_dT2tau_w03(int m, int n, int i, int mt, double *T, int ldt,
 int ora_lila_dV2tau_w03(int m, int n, int i, int mt, double *A, int lda,
                         double *tau);
 
+/* VRT-only (v03), Q from (V,T) (q03), append-columns, LAPACK-style (ker,
+ * w00/z00) and qr2 families (same argument lists as lila-c/lila.h) */
+int ora_lila_dgeqr2_v03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                          double *work, int lwork);
+int ora_lila_dgeqr2_v03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                          double *work, int lwork);
+int ora_lila_dgeqr2_v03(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                        double *T, int ldt, double *work, int lwork);
+int ora_lila_dgeqrf_v03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *work, int lwork);
+int ora_lila_dgeqrf_v03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                             double *work, int lwork);
+int ora_lila_dgeqrf_v03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                           double *T, int ldt, double *work, int lwork);
+int ora_lila_dgeqrf_v03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                  double *T, int ldt, double *work, int lwork);
+int ora_lila_dgeqrf_v03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n,
+                               int i, int mt, double *A, int lda, double *T, int ldt, double *work,
+                               int lwork);
+int ora_lila_dgeqr2_q03_3(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                          double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqr2_q03_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                          double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqr2_q03(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                        double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqrf_q03_mt_hh(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                              double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqrf_q03_mt_l(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                             double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqrf_q03_mt(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                           double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqrf_q03_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                  double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqrf_q03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n,
+                               int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                               int ldq, double *work, int lwork);
+int ora_lila_dgeqrf_w03_appendcols(int *lila_param, int m, int k, int n, int mt, double *A, int lda,
+                                   double *Q, int ldq, double *T, int ldt, double *work, int lwork);
+int ora_lila_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A, int lda, double *Q,
+                                      int ldq, double *T, int ldt, double *work, int lwork);
+int ora_lila_dgeqrf_ker_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                  double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int ora_lila_dormqrf_w00(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                         int ldt, double *work, int lwork);
+int ora_lila_dormqrf_z00(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *B,
+                         int ldb, double *T, int ldt, double *work, int lwork);
+int ora_lila_dormqrf_z02(int m, int n, int k, int i, int j, int mt, double *A, int lda, double *T,
+                         int ldt, double *B, int ldb, double *work, int lwork);
+int ora_lila_dgeqrf_qr2(int *lila_param, int m, int n, int i, int mt, double *A, int lda, double *T,
+                        int ldt, double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqrf_qr2_recursive(int *lila_param, int m, int n, int i, int mt, double *A, int lda,
+                                  double *T, int ldt, double *Q, int ldq, double *work, int lwork);
+int ora_lila_dgeqrf_qr2_recursive2(int *lila_param, int m, int n, int i, int mt, int nb1, int nb2,
+                                   double *A, int lda, double *T, int ldt, double *Q, int ldq,
+                                   double *work, int lwork);
+
 /* workspace queries (return doubles needed; lwork is the running maximum) */
 int ora_lila_query_dormqrf_w03(int m, int n, int k, int i, int j, int mt,
                                double *A, int lda, double *T, int ldt,
