@@ -162,10 +162,12 @@ __device__ void store_slab(const double *slab, int rs, int nb, int nloc, double 
 }
 
 // ---------------------------------------------------------------------------
-// column-by-column Householder on the slab in LDS (+ T, Q, write-back).
-// SOLO: G == 1, reductions stay in LDS.
+// column-by-column Householder on the slab (+ T, Q, write-back).
+// SOLO: G == 1, reductions stay in LDS.  GLOB: the slab is the panel itself
+// in global memory (rows r0.. of A, column stride lda): panels taller than
+// the workgroups' LDS can hold stream every column step through HBM / L2.
 // ---------------------------------------------------------------------------
-template <bool SOLO>
+template <bool SOLO, bool GLOB = false>
 __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0, int nloc,
                           unsigned &nbar) {
   const int nb = a.nb, RS = a.rs, G = a.G;
@@ -348,7 +350,7 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
   }
 
   // ---- write V and R back ----
-  store_slab(slab, RS, nb, nloc, a.A + r0, a.lda, nullptr);
+  if (!GLOB) store_slab(slab, RS, nb, nloc, a.A + r0, a.lda, nullptr);
 
   // ---- T = dlarft('F','C') from tau and the Gram entries ----
   for (int t = tid; t < PNB * PNB; t += PNT) Tl[t] = 0.0;
@@ -448,6 +450,18 @@ __global__ __launch_bounds__(PNT) void panel_hh_kernel(PanelArgs a) {
   unsigned nbar = 0;
   hh_factor<SOLO>(a, slab, tail, r0, nloc, nbar);
   if (!SOLO) exit_protocol(a);
+}
+
+// the same on a panel left in global memory (a.rs == a.lda); LDS holds the
+// fixed part only
+__global__ __launch_bounds__(PNT) void panel_hh_global_kernel(PanelArgs a) {
+  extern __shared__ double smem[];
+  const int g = blockIdx.x;
+  const int r0 = g * a.rp;
+  const int nloc = max(0, min(a.ml, r0 + a.rp) - r0);
+  unsigned nbar = 0;
+  hh_factor<false, true>(a, a.A + r0, smem, r0, nloc, nbar);
+  exit_protocol(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -864,11 +878,10 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   int rp = (ml + G - 1) / G;
   if (rp < nb) rp = nb;
   G = (ml + rp - 1) / rp;
-  if (rp > rmax) {
-    ld_fatal("panel taller than 256 slabs of LDS rows reached the fused panel kernel");
-    return;
-  }
-  const int rs = rp | 1;  // odd LDS column stride
+  // taller than 256 LDS slabs: the column-by-column kernel on the panel in
+  // global memory, 256 workgroups
+  const bool glob = rp > rmax;
+  const int rs = glob ? lda : (rp | 1);  // odd LDS column stride
   // Co-residency of all G workgroups is what the in-kernel barrier needs.
   // It is checked once from the occupancy calculator (one workgroup per CU
   // at the largest LDS size, G <= CU count); a plain launch then keeps all
@@ -884,6 +897,8 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
     hipFuncSetAttribute((const void *)panel_cqr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)maxb);
+    hipFuncSetAttribute((const void *)panel_hh_global_kernel,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
     int dev = 0, per_cu = 0, per_cu2 = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -907,7 +922,10 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   pa.err = flags;
   const void *fn;
   const char *cls;
-  if (G == 1) {
+  if (glob) {
+    fn = (const void *)panel_hh_global_kernel;
+    cls = "panel_hh_global_kernel";
+  } else if (G == 1) {
     fn = (const void *)panel_hh_kernel<true>;
     cls = "panel_hh_kernel<true>";
   } else if (mode == 1) {
@@ -918,7 +936,7 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
     cls = "panel_hh_kernel<false>";
   }
   void *args[] = {&pa};
-  const size_t shb = panel_smem_bytes(nb, rs);
+  const size_t shb = glob ? sizeof(double) * LDS_TAIL : panel_smem_bytes(nb, rs);
   const int ph = ld_prof_begin(cls);
   hipError_t e;
   if (coop && G > 1) {

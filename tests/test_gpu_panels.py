@@ -47,7 +47,8 @@ def run_dgeqr3(lila, ora, A):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("m,n", [(100, 7), (2000, 32), (20000, 32), (126000, 32), (250000, 16)])
+@pytest.mark.parametrize("m,n", [(100, 7), (2000, 32), (20000, 32), (126000, 32), (250000, 16),
+                                 (400000, 32), (1000000, 12)])
 def test_dgeqr3_both_modes(lila, ora, panel_mode, mode, m, n):
     say("dgeqr3 mode", mode, m, n)
     panel_mode(mode)
@@ -66,6 +67,28 @@ def test_recursive_both_modes(lila, ora, panel_mode, mode):
     A = ora.fill_uniform(5, m, n + ii, -0.5, 0.5)
     Q = np.zeros((m, n + ii), order="F")
     T = np.zeros((mt, n + ii), order="F")
+    p = np.array([0, 1, 1, nx], dtype=np.int32)
+    Ao, To, Qo = A.copy(order="F"), T.copy(order="F"), Q.copy(order="F")
+    ora.call("lila_dgeqrf_w03_recursive", p, m, n, ii, mt, Ao, m, To, mt, Qo, m, None, 0)
+    Ad, Td, Qd = to_dev(A), to_dev(T), to_dev(Q)
+    assert lila.call("lila_dgeqrf_w03_recursive", p, m, n, ii, mt, Ad, m, Td, mt, Qd, m, None, 0) == 0
+    torch.cuda.synchronize()
+    assert lila.call("lila_get_info") == 0
+    tol = 100 * n * U64
+    assert rel(to_host(Ad), Ao) < tol
+    assert rel(to_host(Qd), Qo) < tol
+    assert rel(to_host(Td), To) < 10 * tol
+
+
+def test_tall_recursive_beyond_lds_capacity(lila, ora, panel_mode):
+    """m = 300000 puts every 32-column panel past 256 LDS slabs: the panels run
+    the column-by-column kernel on global memory"""
+    m, n, ii, mt, nx = 300000, 96, 0, 64, 32
+    say("tall recursive", m, n)
+    panel_mode(1)
+    A = ora.fill_normal(77, m, n)
+    Q = np.zeros((m, n), order="F")
+    T = np.zeros((mt, n), order="F")
     p = np.array([0, 1, 1, nx], dtype=np.int32)
     Ao, To, Qo = A.copy(order="F"), T.copy(order="F"), Q.copy(order="F")
     ora.call("lila_dgeqrf_w03_recursive", p, m, n, ii, mt, Ao, m, To, mt, Qo, m, None, 0)
