@@ -68,7 +68,11 @@ int lila_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int l
 int lila_dlarft_w03(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
                     double *tau);
 
-/* lila-c/lila.h:46-56 */
+/* lila-c/lila.h:46-56.  lila_param[0] (mode, unused by the reference's
+ * drivers) selects, in lila_dgeqrf_w03_recursive, 0: the FP64 recursion;
+ * 1: mixed precision -- FP32-MFMA Householder R0 as a preconditioner, FP64
+ * Cholesky-QR twice on A R0^-1, Householder reconstruction to the same
+ * V, R, T, Q outputs (lila_param[3] is its recursion cutoff). */
 int lila_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n, int i,
                            int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,
                            double *work, int lwork);
@@ -226,6 +230,12 @@ int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, cons
  * column-by-column kernel as in-kernel fallback; 0 always column-by-column
  * (dgeqr3's arithmetic).  Both give dgeqr3's V, R, T and Q to rounding. */
 int lila_set_panel_mode(int mode);
+
+/* Arithmetic of the library's GEMMs (returns the previous setting): 0
+ * (default) FP64 MFMA; 1 FP32 MFMA with FP32 accumulation, operands rounded
+ * to FP32 (the low-precision updates of the mixed-precision QR, selected by
+ * lila_param[0] == 1 in lila_dgeqrf_w03_recursive; exposed for testing). */
+int lila_set_gemm_lowp(int on);
 
 /* Launch profiling: when on, every launch of a hot kernel is bracketed by
  * hipEvents on the library stream; lila_prof_read writes one line per

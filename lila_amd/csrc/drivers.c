@@ -370,9 +370,52 @@ int dv_dgeqrf_w03_mt(const int *p, int m, int n, int i, int mt, double *A, int l
     return 0;
 }
 
-/* lila-c/lila_dgeqrf_w03_recursive.c:3-29 */
+/* Mixed precision (lila_param[0] == 1): the R0 of a Householder QR whose
+ * trailing updates and T products run on FP32 MFMA (panels in FP64)
+ * preconditions A; FP64 Cholesky-QR twice on A R0^-1 gives an orthonormal Q
+ * and R = R2 R1 R0; the blocked Householder reconstruction (lila_dorghr)
+ * turns (Q, R) into the same V, R, T and Q outputs as the FP64 path, in
+ * exact arithmetic and to FP64 accuracy for kappa(A) up to ~1/(u32 c). */
+int dv_dgeqrf_mixed(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                    double *Q, int ldq) {
+    int ml = m - i;
+    if (n <= 0) return 0;
+    double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
+    size_t mark = ld_ws_mark();
+    double *W = wsd((size_t)ml * n), *T0 = wsd((size_t)mt * n);
+    double *G = wsd((size_t)n * n), *R = wsd((size_t)n * n), *Sd = wsd((size_t)(i + n));
+    /* R0: FP32-MFMA updates, FP64 panels */
+    ld_lacpy('A', ml, n, Aii, lda, W, ml);
+    int p0[4] = {0, 1, 1, p[3] > 1 ? p[3] : 32};
+    ld_tune_t *tu = ld_tune();
+    const int old = tu->gemm_lowp;
+    tu->gemm_lowp = 1;
+    dv_dgeqrf_v03_recursive(p0, ml, n, 0, mt, W, ml, T0, mt);
+    tu->gemm_lowp = old;
+    /* Q1 = A R0^-1 and R = R0 */
+    ld_lacpy('A', ml, n, Aii, lda, Qii, ldq);
+    ld_trsm('R', 'U', 'N', 'N', ml, n, 1.0, W, ml, Qii, ldq);
+    ld_laset('L', n, n, 0.0, 0.0, R, n);
+    ld_lacpy('U', n, n, W, ml, R, n);
+    /* Cholesky-QR twice: G = Q^T Q = Rc^T Rc, Q <- Q Rc^-1, R <- Rc R */
+    for (int pass = 0; pass < 2; pass++) {
+        ld_syrk_ut(n, ml, 1.0, Qii, ldq, 0.0, G, n);
+        ld_potrf_u(n, G, n);
+        ld_trsm('R', 'U', 'N', 'N', ml, n, 1.0, G, n, Qii, ldq);
+        ld_trmm('L', 'U', 'N', 'N', n, n, 1.0, G, n, R, n);
+    }
+    /* Householder reconstruction of (Q, R) in the mt layout */
+    ld_lacpy('U', n, n, R, n, Aii, lda);
+    dv_dorghr(m, n, i, mt, A, lda, T, ldt, Q, ldq, Sd);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* lila-c/lila_dgeqrf_w03_recursive.c:3-29; lila_param[0] == 1 selects the
+ * mixed-precision form above */
 int dv_dgeqrf_w03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
                             int ldt, double *Q, int ldq) {
+    if (p[0] == 1) return dv_dgeqrf_mixed(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
     if (n <= p[3]) return dv_dgeqrf_w03_mt(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
     int n1 = n / 2, n2 = n - n1;
     dv_dgeqrf_w03_recursive(p, m, n1, i, mt, A, lda, T, ldt, Q, ldq);

@@ -17,11 +17,22 @@
 // split over grid.z into FP64 partials that a second kernel reduces in a
 // fixed order (deterministic).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 #include <stdio.h>
 #include "lila_dev.h"
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// one MFMA step on either precision: FP64 v_mfma_f64_16x16x4f64, or FP32
+// v_mfma_f32_16x16x4f32 (twice the rate) for the low-precision updates
+__device__ __forceinline__ d4_t mfma16(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
 namespace {
 
@@ -53,7 +64,9 @@ __device__ __forceinline__ bool tile_needs_mask(int s, int r0, int r1, int c0, i
   return r1 >= c0;                                      // upper forms: some r >= c
 }
 
-template <int BM, int BN, bool TA, bool TB>
+// F32: operands rounded to FP32 when staged into LDS, FP32 MFMA with FP32
+// accumulation, result widened to FP64 in the epilogue (C stays FP64)
+template <int BM, int BN, bool TA, bool TB, bool F32 = false>
 __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
     int M, int N, int K, double alpha, double beta,
     const double *__restrict__ A, int lda, int sa,
@@ -68,8 +81,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
   constexpr int NLA = BM * BK / NT;
   constexpr int NLB = BN * BK / NT;
   // double-buffered tiles, k-major: As[buf][k][m], Bs[buf][k][n]
-  __shared__ double As[2][BK][BM + PA];
-  __shared__ double Bs[2][BK][BN + PB];
+  typedef typename std::conditional<F32, float, double>::type lds_t;
+  typedef typename std::conditional<F32, f4_t, d4_t>::type acc_t;
+  __shared__ lds_t As[2][BK][BM + PA];
+  __shared__ lds_t Bs[2][BK][BN + PB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -96,11 +111,11 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
   const bool a_rows_full = (m0 + BM <= M);
   const bool b_cols_full = (n0 + BN <= N);
 
-  d4_t acc[WMT][WNT];
+  acc_t acc[WMT][WNT];
 #pragma unroll
   for (int a = 0; a < WMT; a++)
 #pragma unroll
-    for (int b = 0; b < WNT; b++) acc[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < WNT; b++) acc[a][b] = acc_t{0, 0, 0, 0};
 
   double ra[NLA], rb[NLB];
 
@@ -146,13 +161,13 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
     for (int t = 0; t < NLA; t++) {
       const int i = TA ? a_i0 + t * (NT / BK) : a_i0;
       const int pk = TA ? a_p0 : a_p0 + t * (NT / BM);
-      As[buf][pk][i] = ra[t];
+      As[buf][pk][i] = (lds_t)ra[t];
     }
 #pragma unroll
     for (int t = 0; t < NLB; t++) {
       const int j = TB ? b_j0 : b_j0 + t * (NT / BK);
       const int pk = TB ? b_p0 + t * (NT / BN) : b_p0;
-      Bs[buf][pk][j] = rb[t];
+      Bs[buf][pk][j] = (lds_t)rb[t];
     }
   };
 
@@ -168,7 +183,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
 #pragma unroll
     for (int kk = 0; kk < BK / 4; kk++) {
       const int kr = kk * 4 + (lane >> 4);
-      double af[WMT], bf[WNT];
+      lds_t af[WMT], bf[WNT];
 #pragma unroll
       for (int a = 0; a < WMT; a++) af[a] = As[buf][kr][wm * (BM / 2) + a * 16 + (lane & 15)];
 #pragma unroll
@@ -177,7 +192,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
       for (int a = 0; a < WMT; a++)
 #pragma unroll
         for (int b = 0; b < WNT; b++)
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[b], af[a], acc[a][b], 0, 0, 0);
+          acc[a][b] = mfma16(bf[b], af[a], acc[a][b]);
     }
     if (more) store_tiles(buf ^ 1);
     __syncthreads();
@@ -185,8 +200,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
   }
 
   // epilogue: the f64 16x16x4 accumulator puts D row (lane/16) + 4v in lane
-  // `lane`, register v (rows interleaved by 4), D column lane%16; D = C^T
-  // here, so lane holds C(row = base + lane%16, col = base + lane/16 + 4v)
+  // `lane`, register v (rows interleaved by 4), D column lane%16; the f32
+  // 16x16x4 one puts row 4 (lane/16) + v there (four consecutive rows).
+  // D = C^T here, so lane holds C(row = base + lane%16, col = base + D row)
   const int rl = lane & 15, cg = lane >> 4;
 #pragma unroll
   for (int a = 0; a < WMT; a++) {
@@ -196,9 +212,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
     for (int b = 0; b < WNT; b++) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int gj = n0 + wn * (BN / 2) + b * 16 + cg + 4 * v;
+        const int gj = n0 + wn * (BN / 2) + b * 16 + (F32 ? 4 * cg + v : cg + 4 * v);
         if (gj >= N || (c_upper && gi > gj)) continue;
-        double r = acc[a][b][v];
+        double r = (double)acc[a][b][v];
         if (P) {
           P[(size_t)blockIdx.z * M * N + (size_t)gi + (size_t)gj * M] = r;
         } else {
@@ -255,14 +271,22 @@ void launch_tile(const ld_gemm_t *g, int splits, int kchunk, double *P, hipStrea
   dim3 grid((g->m + BM - 1) / BM, (g->n + BN - 1) / BN, splits);
   dim3 block(NT);
   char cls[96];
-  snprintf(cls, sizeof cls, "gemm_f64_kernel<%d, %d, %s, %s>", BM, BN, g->ta ? "true" : "false",
-           g->tb ? "true" : "false");
+  const bool f32 = ld_tune()->gemm_lowp != 0;
+  snprintf(cls, sizeof cls, "%s<%d, %d, %s, %s>", f32 ? "gemm_f32mfma_kernel" : "gemm_f64_kernel", BM,
+           BN, g->ta ? "true" : "false", g->tb ? "true" : "false");
   const double M = g->m, N = g->n, K = g->k;
   const int ph = ld_prof_begin(cls);
 #define LGO(TA_, TB_)                                                                      \
-  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA_, TB_>), grid, block, 0, st, g->m, g->n,   \
-                     g->k, g->alpha, g->beta, g->A, g->lda, g->sa, g->B, g->ldb, g->sb,     \
-                     g->C, g->ldc, g->zero_rows, g->c_upper, kchunk, P)
+  do {                                                                                     \
+    if (f32)                                                                               \
+      hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA_, TB_, true>), grid, block, 0, st,    \
+                         g->m, g->n, g->k, g->alpha, g->beta, g->A, g->lda, g->sa, g->B,   \
+                         g->ldb, g->sb, g->C, g->ldc, g->zero_rows, g->c_upper, kchunk, P); \
+    else                                                                                   \
+      hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA_, TB_>), grid, block, 0, st, g->m,    \
+                         g->n, g->k, g->alpha, g->beta, g->A, g->lda, g->sa, g->B, g->ldb,  \
+                         g->sb, g->C, g->ldc, g->zero_rows, g->c_upper, kchunk, P);        \
+  } while (0)
   if (!g->ta && !g->tb) LGO(false, false);
   else if (!g->ta && g->tb) LGO(false, true);
   else if (g->ta && !g->tb) LGO(true, false);

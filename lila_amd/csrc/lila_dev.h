@@ -104,8 +104,6 @@ void ld_fill_uniform(unsigned long long seed, int m, int n, double *A, int ld,
                      double lo, double hi);
 void ld_fill_normal(unsigned long long seed, int m, int n, double *A, int ld);
 
-/* mixed-precision building blocks */
-void ld_gemm_lowp(int mode, const ld_gemm_t *g); /* mode 1: fp32 MFMA, 2: bf16x3 */
 
 /* ---- runtime ------------------------------------------------------------ */
 int  ld_init(void);               /* 0 on success */
@@ -144,6 +142,7 @@ typedef struct {
     int gemm_bm, gemm_bn;    /* default (large) GEMM tile */
     int gemm_small_tiles;    /* below this many large tiles use 64 x 64 */
     int panel_mode;          /* 0 column Householder, 1 Cholesky-QR2 + reconstruction */
+    int gemm_lowp;           /* 1: GEMMs on FP32 MFMA (operands rounded to FP32) */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

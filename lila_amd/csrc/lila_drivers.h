@@ -83,6 +83,9 @@ int dv_dgeqrf_qr2_recursive(const int *p, int m, int n, int i, int mt, double *A
 int dv_dgeqrf_qr2_recursive2(const int *p, int m, int n, int i, int mt, int nb1, int nb2, double *A,
                              int lda, double *T, int ldt, double *Q, int ldq);
 
+int dv_dgeqrf_mixed(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                    double *Q, int ldq);
+
 int dq_dormqrf_w03(int m, int n, int k, int i, int mt);
 int dq_dormqrbz_w03(int m, int n, int k, int i, int mt);
 int dq_dgeqrf_w03_mt(const int *p, int m, int n, int i, int mt, int lwork);
