@@ -53,6 +53,88 @@ def parse_prof(txt):
     return rows
 
 
+def qr_errors(lila_amd, torch, dev, A0, A, Q, m, n):
+    """||A0 - Q R||_F / ||A0||_F and ||I - Q^T Q||_F, R = upper triangle of A"""
+    f64 = torch.float64
+    R = torch.tril(A)  # upper triangle of the column-major A
+    QR = torch.empty_like(A0)
+    lila_amd.call("lila_dgemm", ctypes.c_char(b"N"), ctypes.c_char(b"N"), m, n, n, 1.0, Q, m, R, m,
+                  0.0, QR, m)
+    torch.cuda.synchronize()
+    bwd = (torch.linalg.norm(QR - A0) / torch.linalg.norm(A0)).item()
+    del QR, R
+    G = torch.eye(n, dtype=f64, device=dev)
+    lila_amd.call("lila_dgemm", ctypes.c_char(b"T"), ctypes.c_char(b"N"), n, n, m, 1.0, Q, m, Q, m,
+                  -1.0, G, n)
+    torch.cuda.synchronize()
+    orth = torch.linalg.norm(G).item()
+    return bwd, orth
+
+
+def mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, steps, bwd_plain, orth_plain,
+              dt_plain):
+    """mixed precision (lila_param[0] = 1) on the same matrix, timed like the
+    plain steps and counted with the same VRTQ flops; then both forms on
+    A = U diag(sigma) V^T with kappa in {1e2, 1e5, 1e8} (U, V: Q factors of
+    seeded Gaussians from the plain path, sigma geometric from 1 to 1/kappa)"""
+    pm = np.array([1, 1, 1, nx], dtype=np.int32)
+    pp = np.array([0, 1, 1, nx], dtype=np.int32)
+
+    def run(p, src):
+        A.copy_(src)
+        r = lila_amd.call("lila_dgeqrf_w03_recursive", p, m, n, 0, mt, A, m, T, mt, Q, m, None, 0)
+        if r != 0:
+            raise RuntimeError("lila_dgeqrf_w03_recursive (mode %d) returned %d" % (p[0], r))
+
+    run(pm, A0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run(pm, A0)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    bwd, orth = qr_errors(lila_amd, torch, dev, A0, A, Q, m, n)
+    out = {"value": vrtq_flops(m, n) / dt / 1e12, "unit": "TFLOP/s", "ms_per_step": dt * 1e3,
+           "speedup_vs_plain": dt_plain / dt,
+           "accuracy": {"backward_error": bwd, "orthogonality": orth},
+           "plain_accuracy": {"backward_error": bwd_plain, "orthogonality": orth_plain},
+           "method": "FP32-MFMA Householder R0 (FP64 panels), FP64 Cholesky-QR x2 on A R0^-1, "
+                     "Householder reconstruction (lila_dorghr); flops counted as plain VRTQ",
+           "by_kappa": []}
+    if m != n:
+        return out
+    # U and V from the plain path (Q of seeded Gaussians)
+    f64 = torch.float64
+    G = torch.empty_like(A0)
+    UV = []
+    for seed in (101, 202):
+        lila_amd.call("lila_fill_normal", ctypes.c_ulonglong(seed), m, n, G, m)
+        run(pp, G)
+        UV.append(Q.clone())
+    U, V = UV
+    idx = torch.arange(n, dtype=f64, device=dev)
+    Ak = torch.empty_like(A0)
+    for kappa in (1e2, 1e5, 1e8):
+        sig = kappa ** (-idx / (n - 1))
+        US = U * sig[:, None]  # column j of the column-major U scaled by sigma_j
+        lila_amd.call("lila_dgemm", ctypes.c_char(b"N"), ctypes.c_char(b"T"), m, n, n, 1.0, US, m, V,
+                      n, 0.0, Ak, m)
+        del US
+        row = {"kappa": kappa}
+        for name, p in (("plain", pp), ("mixed", pm)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run(p, Ak)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            b, o = qr_errors(lila_amd, torch, dev, Ak, A, Q, m, n)
+            row[name] = {"backward_error": b, "orthogonality": o, "ms": t * 1e3}
+        out["by_kappa"].append(row)
+    del U, V, G, Ak, UV
+    torch.cuda.empty_cache()
+    return out
+
+
 def read_prof(lib):
     buf = ctypes.create_string_buffer(1 << 16)
     lib.lila_prof_read(buf, len(buf))
@@ -81,15 +163,16 @@ def cpu_baseline(m_target, mt, nx, budget_s):
         pyoracle.call("lila_dgeqrf_w03_recursive", p, nn, nn, 0, mt, A, nn, T, mt, Q, nn, None, 0)
         return time.perf_counter() - t0
 
+    # grow the sample while a run takes under a twentieth of the budget, then
+    # size the timed run from the last measured rate
     nn = 1024
     t = run(nn)
+    while t < budget_s / 20 and nn < m_target:
+        nn = min(m_target, nn * 2)
+        t = run(nn)
     rate = vrtq_flops(nn, nn) / t
-    # pick the size whose run takes about budget_s at the measured rate
-    while nn < m_target:
-        nxt = int(nn * 1.5) // 64 * 64
-        if vrtq_flops(nxt, nxt) / rate > budget_s:
-            break
-        nn = nxt
+    target = int((budget_s * rate / (8.0 / 3.0)) ** (1.0 / 3.0)) // 64 * 64
+    nn = max(nn, min(m_target, target))
     t = run(nn)
     rate = vrtq_flops(nn, nn) / t
     pyoracle.use_builtin_blas()
@@ -119,6 +202,8 @@ def main():
     ap.add_argument("--gemm-tile", default=None, help="GEMM tile override, e.g. 64x64")
     ap.add_argument("--panel-mode", type=int, default=1,
                     help="1: Cholesky-QR2 + reconstruction panel (default), 0: column Householder")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision leg and its per-kappa accuracy table")
     args = ap.parse_args()
     if args.gemm_tile:
         os.environ["LILA_GEMM_TILE"] = args.gemm_tile
@@ -199,6 +284,10 @@ def main():
     orth = torch.linalg.norm(G).item()
     del G
 
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, args.steps, bwd, orth, dt)
+
     roof = None
     if prof:
         dom = max(prof, key=lambda r: r["ms"])
@@ -240,6 +329,7 @@ def main():
                              if args.panel_mode else "dgeqr3 column by column"),
                    "flops_per_step": flops},
         "accuracy": {"backward_error": bwd, "orthogonality": orth, "device_status": info},
+        "mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_classes": prof_all,

@@ -355,9 +355,31 @@ void ld_larft(int ml, int k, const double *V, int ldv, const double *tau, double
   ld_ws_release(mark);
 }
 
+// Sign-chosen unpivoted LU of the mr x nc panel A (mr >= nc), recursive over
+// the columns: the left half, U12 = L11^-1 A12, A22 -= L21 U12 (FP64 MFMA),
+// the right half; column k's negation (S_k = -1) reaches its U12 entries
+// afterwards -- every step before k is linear in column k, so negating the
+// column when it is reached or at the start gives the same factors.  Panels
+// of at most 32 columns run the one-workgroup kernel.
+static void lu_sign_panel(int mr, int nc, double *A, int lda, double *Sg, int rule) {
+  if (nc <= 0) return;
+  if (nc <= 32) {
+    hipLaunchKernelGGL(lu_sign_kernel, dim3(1), dim3(1024), 0, S(), nc, mr, A, lda, Sg, rule);
+    return;
+  }
+  const int n1 = (nc / 2 + 31) / 32 * 32, n2 = nc - n1;
+  lu_sign_panel(mr, n1, A, lda, Sg, rule);
+  double *A12 = A + (size_t)n1 * lda, *A21 = A + n1, *A22 = A + n1 + (size_t)n1 * lda;
+  ld_trsm('L', 'L', 'N', 'U', n1, n2, 1.0, A, lda, A12, lda);
+  ld_gemm_simple(0, 0, mr - n1, n2, n1, -1.0, A21, lda, LD_GEN, A12, lda, LD_GEN, 1.0, A22, lda);
+  lu_sign_panel(mr - n1, n2, A22, lda, Sg + n1, rule);
+  hipLaunchKernelGGL(flip_where_kernel, dim3((n1 + 255) / 256, n2), dim3(256), 0, S(), n1, n2, A12, lda,
+                     Sg + n1, -1.0);
+}
+
 void ld_lu_sign(int n, double *Tq, int ldt, double *Sg) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(lu_sign_kernel, dim3(1), dim3(1024), 0, S(), n, n, Tq, ldt, Sg, 0);
+  lu_sign_panel(n, n, Tq, ldt, Sg, 0);
 }
 
 // c3/lila_dgetrf_b.c semantics: A (ml x n) holds the orthonormal basis, Q
@@ -368,7 +390,7 @@ void ld_getrf_b(int ml, int n, double *A, int lda, double *Q, int ldq, double *U
   if (n <= 0 || ml <= 0) return;
   size_t mark = ld_ws_mark();
   double *Sg = (double *)ld_ws_alloc(sizeof(double) * (size_t)n);
-  hipLaunchKernelGGL(lu_sign_kernel, dim3(1), dim3(1024), 0, S(), n, n, A, lda, Sg, 1);
+  lu_sign_panel(n, n, A, lda, Sg, 1);
   if (ml > n) {
     hipLaunchKernelGGL(flip_where_kernel, dim3((ml - n + 255) / 256, n), dim3(256), 0, S(), ml - n, n,
                        A + n, lda, Sg, -1.0);
