@@ -220,6 +220,33 @@ void *lila_get_stream(void);
  * first n rows. */
 int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb);
 
+/* Batched least squares over nprob independent problems of uneven sizes:
+ * problem p has A_p (m[p] x n[p], lda[p]) and B_p (m[p] x nrhs[p], ldb[p]).
+ * On exit, as LAPACK dgels('N') leaves them: A_p holds R on and above the
+ * diagonal and the Householder vectors (dlarfg convention, unit diagonal
+ * implied) below it; rows [0, n) of B_p hold the solution X, rows [n, m)
+ * the rest of Q^T B (their column norms are the residual norms).  The
+ * per-problem sequence is that of least-squares/main_ls_qr3.c:94-119
+ * (factor, apply Q^T, back-substitute R); the batch runs as one persistent
+ * launch with a work queue ordered by flop count.
+ * Limits: m >= n >= 1, n <= 128, 0 <= nrhs <= 16; returns -1 (with a
+ * message naming the problem) otherwise.  All-device pointers run
+ * asynchronously on the library stream; all-host pointers are staged. */
+int lila_dgels_batched(int nprob, const int *m, const int *n, const int *nrhs, double *const *A,
+                       const int *lda, double *const *B, const int *ldb);
+/* The same as a reusable plan over device pointers (the problem table is
+ * uploaded once): run(use_graph = 1) replays a captured hipGraph of the
+ * launch, run(0) launches it directly; flops() is the plan's flop count
+ * (2mn^2 - n^3/3 + 4(m-n)nk + 4n^2 k summed over the problems, the count
+ * least-squares/main_ls_qr3.c:125-129 reports). */
+typedef struct ld_batch_plan lila_batch_plan;
+lila_batch_plan *lila_dgels_batched_plan(int nprob, const int *m, const int *n, const int *nrhs,
+                                         double *const *A, const int *lda, double *const *B,
+                                         const int *ldb);
+int lila_dgels_batched_run(lila_batch_plan *plan, int use_graph);
+double lila_dgels_batched_flops(const lila_batch_plan *plan);
+void lila_dgels_batched_destroy(lila_batch_plan *plan);
+
 /* C = alpha op(A) op(B) + beta C on FP64 MFMA, dgemm argument order
  * (transa/transb 'N' or 'T'); the GEMM every BLAS-3 step above runs on. */
 int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double *A,

@@ -29,6 +29,11 @@ def load():
             raise ImportError("lila_amd: %s is not built (run __graft_entry__.build())" % LIB_PATH)
         _lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
         _lib.lila_get_stream.restype = ctypes.c_void_p
+        _lib.lila_dgels_batched_plan.restype = ctypes.c_void_p
+        _lib.lila_dgels_batched_flops.restype = ctypes.c_double
+        _lib.lila_dgels_batched_flops.argtypes = [ctypes.c_void_p]
+        _lib.lila_dgels_batched_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _lib.lila_dgels_batched_destroy.argtypes = [ctypes.c_void_p]
     return _lib
 
 

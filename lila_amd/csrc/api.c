@@ -514,6 +514,71 @@ int lila_sync(void) {
     return ld_get_info();
 }
 
+/* ---- batched least squares -------------------------------------------- */
+
+lila_batch_plan *lila_dgels_batched_plan(int nprob, const int *m, const int *n, const int *nrhs,
+                                         double *const *A, const int *lda, double *const *B,
+                                         const int *ldb) {
+    if (ld_init() != 0) {
+        fprintf(stderr, "lila_hip: no usable HIP device; this library has no CPU path\n");
+        return NULL;
+    }
+    for (int p = 0; p < nprob; p++) {
+        if (!ld_is_device_ptr(A[p]) || (nrhs[p] > 0 && !ld_is_device_ptr(B[p]))) {
+            fprintf(stderr, "lila_hip: batched plan needs device pointers (problem %d)\n", p);
+            return NULL;
+        }
+    }
+    return ld_batch_plan_create(nprob, m, n, nrhs, A, lda, B, ldb);
+}
+
+int lila_dgels_batched_run(lila_batch_plan *plan, int use_graph) {
+    if (!plan) return -1;
+    return ld_batch_plan_run(plan, use_graph);
+}
+
+double lila_dgels_batched_flops(const lila_batch_plan *plan) { return ld_batch_plan_flops(plan); }
+
+void lila_dgels_batched_destroy(lila_batch_plan *plan) { ld_batch_plan_destroy(plan); }
+
+int lila_dgels_batched(int nprob, const int *m, const int *n, const int *nrhs, double *const *A,
+                       const int *lda, double *const *B, const int *ldb) {
+    BEGIN();
+    if (nprob == 0) return 0;
+    if (nprob < 0) return -1;
+    int ndev = 0;
+    for (int p = 0; p < nprob; p++) ndev += ld_is_device_ptr(A[p]);
+    if (ndev != 0 && ndev != nprob) {
+        fprintf(stderr, "lila_hip: lila_dgels_batched mixes host and device pointers\n");
+        return -1;
+    }
+    stage_t *sa = (stage_t *)calloc((size_t)nprob, sizeof(stage_t));
+    stage_t *sb = (stage_t *)calloc((size_t)nprob, sizeof(stage_t));
+    double **da = (double **)malloc(sizeof(double *) * (size_t)nprob);
+    double **db = (double **)malloc(sizeof(double *) * (size_t)nprob);
+    int r = -1;
+    if (!sa || !sb || !da || !db) goto done;
+    for (int p = 0; p < nprob; p++) {
+        sa[p] = stage_in(A[p], lda[p], m[p], n[p]);
+        sb[p] = stage_in(B[p], ldb[p], m[p], nrhs[p]);
+        da[p] = sa[p].dev;
+        db[p] = sb[p].dev;
+    }
+    ld_batch_plan *pl = ld_batch_plan_create(nprob, m, n, nrhs, da, lda, db, ldb);
+    if (pl) {
+        r = ld_batch_plan_run(pl, 0);
+        ld_batch_plan_destroy(pl);  /* waits for the launch */
+    }
+    for (int p = 0; p < nprob; p++) {
+        stage_out(&sa[p], r == 0);
+        stage_out(&sb[p], r == 0);
+    }
+    if (r == 0) r = api_end();
+done:
+    free(sa); free(sb); free(da); free(db);
+    return r;
+}
+
 void lila_set_stream(void *s) { ld_set_stream(s); }
 void *lila_get_stream(void) { return ld_stream(); }
 

@@ -146,6 +146,16 @@ typedef struct {
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 
+/* batched least squares (batched.hip): one persistent launch over many
+ * independent problems; the plan holds the problem table in HBM */
+typedef struct ld_batch_plan ld_batch_plan;
+ld_batch_plan *ld_batch_plan_create(int nprob, const int *m, const int *n, const int *k,
+                                    double *const *A, const int *lda, double *const *B,
+                                    const int *ldb);
+int ld_batch_plan_run(ld_batch_plan *pl, int use_graph);
+double ld_batch_plan_flops(const ld_batch_plan *pl);
+void ld_batch_plan_destroy(ld_batch_plan *pl);
+
 #ifdef __cplusplus
 }
 #endif

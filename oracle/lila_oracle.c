@@ -1559,6 +1559,21 @@ int ora_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt,
     return 0;
 }
 
+/* LAPACK DGELS('N') for m >= n, the per-problem host baseline of the batched
+ * least-squares workload (least-squares/main_lapack_ref.c:100 calls dgels):
+ * dgeqr2 (A <- V below the diagonal, R on and above), B <- Q^T B (dorm2r),
+ * then R X = B(0:n, :) by back substitution (the dtrsm of
+ * least-squares/main_ls_qr3.c:117).  tau (n doubles) is returned too. */
+int ora_dgels_hh(int m, int n, int k, double *A, int lda, double *B, int ldb, double *tau) {
+    if (m < n || n < 0 || k < 0) return -1;
+    ora_dgeqr2(m, n, A, lda, tau);
+    if (k > 0) {
+        o_dorm2r('T', m, k, n, A, lda, tau, B, ldb);
+        ora_dtrsm('L', 'U', 'N', 'N', n, k, 1.0, A, lda, B, ldb);
+    }
+    return 0;
+}
+
 /* ---- checkers ------------------------------------------------------------ */
 
 /* ||triu(Q^T Q - I)||_F, the half norm of

@@ -197,6 +197,11 @@ int ora_lila_query_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl,
 int ora_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt,
                double *B, int ldb, double *work);
 
+/* LAPACK dgels('N'), m >= n, as dgeqr2 + dorm2r + back substitution: the
+ * per-problem host baseline of the batched least-squares workload
+ * (least-squares/main_lapack_ref.c:100).  tau: n doubles out. */
+int ora_dgels_hh(int m, int n, int k, double *A, int lda, double *B, int ldb, double *tau);
+
 /* ---- checkers (lila-c-lila_param/lila/src/lila_test_*.c) -------------- */
 double ora_test_qq_orth_1(int m, int n, int i, const double *Q, int ldq);
 double ora_test_qq_orth_full(int m, int n, int i, const double *Q, int ldq);
