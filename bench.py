@@ -187,6 +187,164 @@ def cpu_baseline(m_target, mt, nx, budget_s):
     }
 
 
+def batch_sizes(seed, nprob):
+    """the batched workload's problem sizes: m ~ U{64..4096}, n ~ U{8..min(128, m)},
+    nrhs ~ U{1..16}"""
+    rng = np.random.default_rng(seed)
+    m = rng.integers(64, 4097, size=nprob)
+    n = np.array([rng.integers(8, min(128, int(mm)) + 1) for mm in m])
+    k = rng.integers(1, 17, size=nprob)
+    return m.astype(np.int32), n.astype(np.int32), k.astype(np.int32)
+
+
+def ls_flops(m, n, k):
+    """least-squares/main_ls_qr3.c:125-129"""
+    m, n, k = (np.asarray(x, dtype=np.float64) for x in (m, n, k))
+    return 2.0 * m * n * n - n ** 3 / 3.0 + 4.0 * (m - n) * n * k + 4.0 * n * n * k
+
+
+def batch_cpu_baseline(m, n, k, budget_s, seed):
+    """oracle dgels (dgeqr2 + dorm2r + back substitution, built-in loops) per
+    problem, one problem per thread, on a random sample of the batch"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from concurrent.futures import ThreadPoolExecutor
+    pyoracle.lib()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(m))
+
+    def one(p):
+        mm, nn, kk = int(m[p]), int(n[p]), int(k[p])
+        A = np.asfortranarray(rng_local(p).standard_normal((mm, nn)))
+        B = np.asfortranarray(rng_local(p + 1).standard_normal((mm, kk)))
+        tau = np.zeros(nn)
+        t0 = time.perf_counter()
+        pyoracle.call("dgels_hh", mm, nn, kk, A, mm, B, mm, tau)
+        return time.perf_counter() - t0
+
+    def rng_local(p):
+        return np.random.default_rng(1000003 * seed + p)
+
+    done, busy = 0, 0.0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        chunk = threads * 4
+        while done < len(order) and time.perf_counter() - t0 < budget_s:
+            busy += sum(ex.map(one, order[done:done + chunk]))
+            done += len(order[done:done + chunk])
+    wall = time.perf_counter() - t0
+    # rate from the summed per-problem call times spread over the threads
+    # (input generation in Python is not counted)
+    rate = done / (busy / threads)
+    return {
+        "value": rate,
+        "unit": "QR/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": ("oracle dgels('N') per problem (C restatement of LAPACK dgeqr2 + dorm2r + "
+                   "back substitution, unblocked, built-in loops), %d threads, one problem per "
+                   "thread, %d of the %d problems drawn at random, %.1f s wall"
+                   % (threads, done, len(order), wall)),
+    }
+
+
+def batch_leg(lila_amd, lib, torch, dev, steps, warmup, cpu_seconds, no_cpu, nprob=4096):
+    """4096 uneven least-squares problems (m ~ U{64..4096}, n ~ U{8..128},
+    m >= n, nrhs ~ U{1..16}, N(0,1) entries) in one persistent launch replayed
+    from a hipGraph; QR/s = problems / time of the launch (inputs resident in
+    HBM, restored from a pristine copy outside the timed region)."""
+    seed = 20241204
+    m, n, k = batch_sizes(seed, nprob)
+    lda = m.copy()
+    ldb = m.copy()
+    offA = np.zeros(nprob, dtype=np.int64)
+    offB = np.zeros(nprob, dtype=np.int64)
+    off = 0
+    for p in range(nprob):
+        offA[p] = off
+        off += int(m[p]) * int(n[p])
+        offB[p] = off
+        off += int(m[p]) * int(k[p])
+    buf0 = torch.empty(off, dtype=torch.float64, device=dev)
+    chunk = 1 << 28
+    for c0 in range(0, off, chunk):
+        c1 = min(off, c0 + chunk)
+        lila_amd.call("lila_fill_normal", ctypes.c_ulonglong(seed + c0), c1 - c0, 1, buf0[c0:], c1 - c0)
+    buf = buf0.clone()
+    base = buf.data_ptr()
+    pa = (base + 8 * offA).astype(np.uint64)
+    pb = (base + 8 * offB).astype(np.uint64)
+    plan = lib.lila_dgels_batched_plan(*[lila_amd.conv(a) for a in (nprob, m, n, k, pa, lda, pb, ldb)])
+    if not plan:
+        raise RuntimeError("lila_dgels_batched_plan failed")
+    plan = ctypes.c_void_p(plan)
+    flops = float(ls_flops(m, n, k).sum())
+    times = []
+    lib.lila_prof_only(b"batched_ls_kernel")
+    for it in range(warmup + steps):
+        buf.copy_(buf0)
+        torch.cuda.synchronize()
+        if it == warmup:
+            lib.lila_prof_enable(1)
+        t0 = time.perf_counter()
+        if lib.lila_dgels_batched_run(plan, 1) != 0:
+            raise RuntimeError("lila_dgels_batched_run failed")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if it >= warmup:
+            times.append(t1 - t0)
+    prof = read_prof(lib)
+    lib.lila_prof_enable(0)
+    lib.lila_prof_only(None)
+    info = lila_amd.call("lila_get_info")
+    dt = sum(times) / len(times)
+    # accuracy on a sample of problems: least-squares/main_ls_qr3.c:157-168's
+    # ||A^T r|| / (||A|| (||A|| ||x|| + ||b||)) and R's representation error
+    out = buf.cpu().numpy()
+    src = buf0.cpu().numpy()
+    rng = np.random.default_rng(seed + 1)
+    worst_orth, worst_rep = 0.0, 0.0
+    for p in rng.choice(nprob, size=64, replace=False):
+        mm, nn, kk = int(m[p]), int(n[p]), int(k[p])
+        A = src[offA[p]:offA[p] + mm * nn].reshape((mm, nn), order="F")
+        B = src[offB[p]:offB[p] + mm * kk].reshape((mm, kk), order="F")
+        F = out[offA[p]:offA[p] + mm * nn].reshape((mm, nn), order="F")
+        X = out[offB[p]:offB[p] + mm * kk].reshape((mm, kk), order="F")[:nn]
+        r = A @ X - B
+        nA = np.linalg.norm(A)
+        worst_orth = max(worst_orth, np.linalg.norm(A.T @ r) / (nA * (nA * np.linalg.norm(X) + np.linalg.norm(B))))
+        R = np.triu(F[:nn])
+        # ||A^T A - R^T R|| / ||A||^2 checks R without forming Q
+        worst_rep = max(worst_rep, np.linalg.norm(A.T @ A - R.T @ R) / nA ** 2)
+    lib.lila_dgels_batched_destroy(plan)
+    del buf, buf0
+    kern = None
+    if prof:
+        d = prof[0]
+        avg = d["ms"] / d["launches"]
+        kern = {"kernel": d["name"], "launches": d["launches"], "avg_launch_ms": avg,
+                "tflops": d["flops"] / (d["ms"] * 1e-3) / 1e12,
+                "frac_fp64_mfma_peak": d["flops"] / (d["ms"] * 1e-3) / 1e12 / FP64_MFMA_PEAK}
+    cpu = None
+    if not no_cpu:
+        cpu = batch_cpu_baseline(m, n, k, cpu_seconds, seed)
+    return {
+        "workload": ("batched uneven least squares (lila_dgels_batched plan, hipGraph replay): "
+                     "%d problems, m ~ U{64..4096}, n ~ U{8..min(128,m)}, nrhs ~ U{1..16}, "
+                     "N(0,1)" % nprob),
+        "qr_per_s": nprob / dt,
+        "ms_per_step": dt * 1e3,
+        "tflops": flops / dt / 1e12,
+        "flops_per_step": flops,
+        "bytes_in": 8.0 * off,
+        "kernel": kern,
+        "accuracy": {"max_norm_orth": worst_orth, "max_gram_repres": worst_rep, "sampled": 64,
+                     "device_status": info},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +360,8 @@ def main():
     ap.add_argument("--gemm-tile", default=None, help="GEMM tile override, e.g. 64x64")
     ap.add_argument("--panel-mode", type=int, default=1,
                     help="1: Cholesky-QR2 + reconstruction panel (default), 0: column Householder")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched least-squares leg")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched least-squares leg")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision leg and its per-kappa accuracy table")
     args = ap.parse_args()
@@ -217,6 +377,10 @@ def main():
     lib = lila_amd.load()
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     dev = torch.device("cuda:0")
+    if args.batch_only:
+        print(json.dumps(batch_leg(lila_amd, lib, torch, dev, args.steps, args.warmup,
+                                   args.cpu_seconds, args.no_cpu)))
+        return
     m, n, mt, nx = args.m, args.n, args.mt, args.nx
     f64 = torch.float64
     # column-major m x n storage: torch shape (n, m)
@@ -288,6 +452,11 @@ def main():
     if not args.no_mixed:
         mixed = mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, args.steps, bwd, orth, dt)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(lila_amd, lib, torch, dev, args.steps, args.warmup, args.cpu_seconds,
+                          args.no_cpu)
+
     roof = None
     if prof:
         dom = max(prof, key=lambda r: r["ms"])
@@ -330,6 +499,7 @@ def main():
                    "flops_per_step": flops},
         "accuracy": {"backward_error": bwd, "orthogonality": orth, "device_status": info},
         "mixed": mixed,
+        "batch": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_classes": prof_all,

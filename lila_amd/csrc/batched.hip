@@ -1,21 +1,33 @@
 // batched.hip -- many independent least-squares problems in one persistent
-// launch (the LS sequence of least-squares/main_ls_qr3.c:94-119 per problem:
-// Householder QR with dlarfg's convention, Q^T B through the block
-// reflectors, back substitution with R).
+// launch: per problem the sequence of least-squares/main_ls_qr3.c:94-119
+// (Householder QR with dlarfg's convention, Q^T B through the block
+// reflectors, back substitution with R), outputs in LAPACK dgels' layout.
 //
-// One workgroup (256 threads, 4 waves) owns one problem at a time and takes
-// the next from a queue ordered by cost (largest first), so uneven sizes
-// balance across the CUs.  Per problem, panels of NB = 16 columns:
-//   1. the panel, column by column (one pass for the norm and the dots with
-//      the later panel columns, one pass for the reflector update), on the
-//      matrix in global memory;
-//   2. W = V^T [V, A_trailing, B] on v_mfma_f64_16x16x4f64, the rows staged
-//      through LDS 32 at a time with column-contiguous loads (the first 16
-//      columns of W are the Gram V^T V the T factor needs);
-//   3. T (dlarft 'F','C') from the Gram and tau; W2 = T^T W;
-//   4. [A_trailing, B] -= V W2, again on MFMA with the tile staged in LDS.
-// After the last panel, R X = (Q^T B)(0:n) is solved in place.
-// Limits: m >= n, n <= 128, nrhs <= 16 (checked by the host).
+// One workgroup (512 threads, 8 waves) owns one problem at a time and takes
+// the next from a queue ordered by flop count (largest first), so uneven
+// sizes balance across the CUs.  A problem is factored in panels of 32
+// columns; the trailing columns of A and the columns of B are carried
+// together ("trailing" below).  Every pass streams the rows below the
+// panel's top in chunks of 32 rows through LDS (column-contiguous loads,
+// the next chunk prefetched into registers while the current one is used)
+// and does its arithmetic on v_mfma_f64_16x16x4f64:
+//   a. G1 = P^T P for the first panel P (later panels get G1 from pass U);
+//   b. Cholesky G1 = R1^T R1 (one wave), Q1 = P R1^-1 and G2 = Q1^T Q1;
+//      Cholesky G2 = R2^T R2, R = R2 R1, the top block of Q = Q1 R2^-1;
+//      Householder reconstruction: sign-chosen LU Q_top S - I = L U,
+//      V1 = L, R_hh = S R, T = -U V1^-T;
+//   d. V2 = P2 (R^-1 S U^-1) written in place, and W = V^T [V | trailing]
+//      accumulated in the same pass; W2 = T^T W;
+//   U. trailing -= V W2, and G1 of the next panel accumulated from the
+//      updated columns in the same pass.
+// So each panel costs two reads of the panel and two reads plus one write
+// of the trailing block.  If a Cholesky pivot is not safely positive (or
+// R's diagonal spreads by more than 1e5, or a reflector has tau within
+// 1e-10 of 0 or 2) the panel is factored column by column instead (dlarfg
+// and the reflector update on the panel in global memory) and T comes from
+// dlarft over the Gram V^T V of pass d.  After the last panel R X =
+// (Q^T B)(0:n) is solved in blocks of 32 rows.
+// Limits: m >= n >= 1, n <= 128, 0 <= nrhs <= 16 (checked by the host).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -24,22 +36,70 @@
 #include <vector>
 #include <algorithm>
 #include "lila_dev.h"
+#include "small32.h"
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int BT = 256;              // threads per workgroup
-constexpr int NB = 16;               // panel width
-constexpr int RC = 32;               // rows per staged chunk
+using small32::LDM;
+using small32::PNB;
+
+constexpr int BT = 512;                 // threads per workgroup
+constexpr int NW = BT / 64;             // waves
+constexpr int NB = 32;                  // panel width
+constexpr int RC = 64;                  // rows per staged chunk
 constexpr int MAXN = 128, MAXK = 16;
-constexpr int MAXC = NB + MAXN + MAXK;  // 160 >= jb + trailing A columns + B columns
-constexpr int LC = MAXC + 1;            // LDS row stride of the staged chunk (odd)
+constexpr int MAXT = MAXN - NB + MAXK;  // 112 trailing columns at most
+constexpr int CW = NB + MAXT;           // 144 chunk columns
+constexpr int LC = CW + 1;              // LDS row stride of the chunk (odd)
+constexpr int QS = BT / RC;             // 8: column step between a thread's elements
+constexpr int PF = RC * CW / BT;        // 18 staged elements per thread
+constexpr int UP = NB / QS;             // 4 of them in the panel columns
+static_assert(NB == PNB && RC * CW % BT == 0 && NB % QS == 0 && (RC / 16) * 2 == NW, "chunk mapping");
 
 struct BProb {
   int m, n, k, lda, ldb, pad;
   double *A, *B;
 };
+
+struct Smem {
+  double C[RC][LC];     // chunk rows: panel columns 0..31, trailing columns from 32
+  double W[NB][CW];     // V^T [V | trailing]; the trailing part becomes T^T W
+  double M0[PNB][LDM];  // R1, then R = R2 R1
+  double M2[PNB][LDM];  // R2
+  double M3[PNB][LDM];  // top block of Q, then its LU (U upper, L strict lower)
+  double MX[PNB][LDM];  // transform of a row pass: R1^-1, then R^-1 S U^-1
+  double TT[PNB][LDM];  // R2^-1 / U^-1 scratch, then T
+  double D0[PNB], D2[PNB], D3[PNB], Sv[PNB], tau[PNB], wv[PNB];
+  double red[NW][PNB + 2];
+  double sc[4];
+  int job, flag;
+};
+
+// the panel being factored: first column j0, width jb, na A columns right
+// of it, nt = na + k trailing columns
+struct Pan {
+  int j0, jb, na, nt;
+};
+
+// thread index the compiler cannot hoist (address and mask computations
+// derived from it stay next to their use instead of living, spilled, for the
+// whole persistent kernel)
+__device__ __forceinline__ int otid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+#define THREAD_IDS                                   \
+  const int tid = otid(), lane = tid & 63, wave = tid >> 6; \
+  (void)lane;                                        \
+  (void)wave
+
+__device__ __forceinline__ d4_t mfma(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -52,151 +112,418 @@ __device__ __forceinline__ double wmax(double v) {
   return v;
 }
 
-struct Smem {
-  double C[RC][LC];         // staged rows: [V | A trailing | B], zero padded
-  double W[NB][MAXC];       // V^T [V A B], then T^T W in the A/B part
-  double T[NB][NB];
-  double red[4][NB + 2];    // per-wave partial sums
-  double tau[NB];
-  double w[NB];             // tau * v^T a_q for the panel update
-  double sc[4];             // beta, tau, 1/(alpha-beta)
-  int job;
-};
-
-// element (r, q) of the staged column set: q < jb a reflector of the panel
-// (unit lower structure), then the trailing A columns, then B's columns
-__device__ __forceinline__ double col_val(const BProb &p, int j0, int jb, int na, int r, int q) {
-  if (q < jb) {
-    const int c = j0 + q;
-    if (r < c) return 0.0;
-    if (r == c) return 1.0;
-    return p.A[(size_t)r + (size_t)c * p.lda];
+// ---- chunk staging: thread element u is row tid % RC of the chunk and
+// chunk column tid / RC + QS u (u < UP: panel columns, then trailing) ----
+__device__ __forceinline__ void fetch(const BProb &p, const Pan &P, int rb, bool trail,
+                                      double (&reg)[PF]) {
+  const int tid = otid(), i = tid % RC, q0 = tid / RC, r = rb + i;
+  const bool rok = r < p.m;
+#pragma unroll
+  for (int u = 0; u < PF; u++) {
+    const int q = q0 + QS * u;
+    bool ok;
+    const double *src;
+    if (u < UP) {
+      ok = q < P.jb;
+      src = p.A + (size_t)(P.j0 + q) * p.lda;
+    } else {
+      const int t = q - NB;
+      ok = trail && t < P.nt;
+      src = (t < P.na) ? p.A + (size_t)(P.j0 + P.jb + t) * p.lda
+                       : p.B + (size_t)(t - P.na) * p.ldb;
+    }
+    reg[u] = (rok && ok) ? src[r] : 0.0;
   }
-  q -= jb;
-  if (q < na) return p.A[(size_t)r + (size_t)(j0 + jb + q) * p.lda];
-  q -= na;
-  return p.B[(size_t)r + (size_t)q * p.ldb];
 }
 
-__device__ void solve_problem(const BProb &p, Smem &s) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m = p.m, n = p.n, k = p.k, lda = p.lda;
-  double *A = p.A;
-  for (int j0 = 0; j0 < n; j0 += NB) {
-    const int jb = min(NB, n - j0);
-    // ---------------- 1. panel, column by column ----------------
-    for (int c = 0; c < jb; c++) {
-      const int col = j0 + c, nq = jb - c - 1;
-      double ss = 0.0, mx = 0.0, d[NB - 1];
+// vmask: the chunk holds the panel's top rows and its panel columns are V,
+// stored with R above the diagonal: give them V1's unit lower structure
+__device__ __forceinline__ void put(Smem &s, const double (&reg)[PF], bool vmask, int jb) {
+  const int tid = otid(), i = tid % RC, q0 = tid / RC;
 #pragma unroll
-      for (int q = 0; q < NB - 1; q++) d[q] = 0.0;
+  for (int u = 0; u < PF; u++) {
+    const int q = q0 + QS * u;
+    double v = reg[u];
+    if (u < UP && vmask) v = (i < q) ? 0.0 : (i == q ? (q < jb ? 1.0 : 0.0) : v);
+    s.C[i][q] = v;
+  }
+}
+
+__device__ __forceinline__ void store_trail(const Smem &s, const BProb &p, const Pan &P, int rb) {
+  const int tid = otid(), i = tid % RC, q0 = tid / RC, r = rb + i;
+  if (r >= p.m) return;
+#pragma unroll
+  for (int u = UP; u < PF; u++) {
+    const int t = q0 + QS * u - NB;
+    if (t < P.nt) {
+      double *dst = (t < P.na) ? p.A + (size_t)(P.j0 + P.jb + t) * p.lda
+                               : p.B + (size_t)(t - P.na) * p.ldb;
+      dst[r] = s.C[i][NB + t];
+    }
+  }
+}
+
+// rows rlo.. of the chunk
+__device__ __forceinline__ void store_panel(const Smem &s, const BProb &p, const Pan &P, int rb,
+                                            int rlo) {
+  const int tid = otid(), i = tid % RC, q0 = tid / RC, r = rb + i;
+  if (r >= p.m || i < rlo) return;
+#pragma unroll
+  for (int u = 0; u < UP; u++) {
+    const int q = q0 + QS * u;
+    if (q < P.jb) p.A[(size_t)r + (size_t)(P.j0 + q) * p.lda] = s.C[i][q];
+  }
+}
+
+// ---- 32 x 32 pieces ----
+// Gram tile tt (0: (0,0), 1: (0,1), 2: (1,1)) of X^T X, X = chunk columns
+// c0..c0+31, over the chunk rows from ilo (a multiple of 4) on
+__device__ __forceinline__ d4_t gram_tile(const Smem &s, int c0, int tt, d4_t acc, int ilo = 0) {
+  const int lane = otid() & 63;
+  const int ti = (tt == 2) ? 1 : 0, tj = (tt == 0) ? 0 : 1;
+#pragma unroll
+  for (int ks = 0; ks < RC / 4; ks++) {
+    const int i = 4 * ks + (lane >> 4);
+    if (4 * ks >= ilo)
+      acc = mfma(s.C[i][c0 + ti * 16 + (lane & 15)], s.C[i][c0 + tj * 16 + (lane & 15)], acc);
+  }
+  return acc;
+}
+__device__ __forceinline__ void gram_store(double (*M)[LDM], int tt, d4_t acc) {
+  const int lane = otid() & 63;
+  const int ti = (tt == 2) ? 1 : 0, tj = (tt == 0) ? 0 : 1;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    M[ti * 16 + (lane >> 4) + 4 * v][tj * 16 + (lane & 15)] = acc[v];
+    if (tt == 1) M[16 + (lane >> 4) + 4 * v][lane & 15] = 0.0;
+  }
+}
+// identity beyond jb on the diagonal (the data there is zero)
+__device__ __forceinline__ void pad_identity(double (*M)[LDM], int jb) {
+  const int tid = otid();
+  if (tid < PNB && tid >= jb) M[tid][tid] = 1.0;
+}
+
+// X = R^-1 for upper R with D = 1 / diag(R): thread c < 32 forms column c
+__device__ __forceinline__ void inv_upper(const double (*R)[LDM], const double *D,
+                                          double (*X)[LDM], int c) {
+  double x[PNB];
+#pragma unroll
+  for (int i = PNB - 1; i >= 0; i--) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = i + 1; q < PNB; q++) acc += R[i][q] * x[q];
+    x[i] = (i < c) ? -D[i] * acc : (i == c ? D[c] : 0.0);
+  }
+#pragma unroll
+  for (int i = 0; i < PNB; i++) X[i][c] = x[i];
+}
+
+// (X Y)(r, c) for upper X, Y; colscale multiplies X's column q by sc[q]
+__device__ __forceinline__ double upper_prod(const double (*X)[LDM], const double (*Y)[LDM],
+                                             const double *sc, int r, int c) {
+  double acc = 0.0;
+  for (int q = r; q <= c; q++) acc += X[r][q] * (sc ? sc[q] : 1.0) * Y[q][c];
+  return acc;
+}
+
+// ---- column-by-column fallback: dlarfg and the reflector update on the
+// panel in global memory; tau into s.tau ----
+// (scalar arguments: a struct passed by reference to an outlined call would
+// be kept in scratch memory by the caller)
+__device__ __noinline__ void panel_columns(double *A, int m, int lda, int j0, int jb, Smem &s) {
+  const int tid = otid(), lane = tid & 63, wave = tid >> 6;
+  for (int c = 0; c < jb; c++) {
+    const int col = j0 + c, nq = jb - c - 1;
+    double ss = 0.0, mx = 0.0, d[NB - 1];
+#pragma unroll
+    for (int q = 0; q < NB - 1; q++) d[q] = 0.0;
+    for (int r = col + 1 + tid; r < m; r += BT) {
+      const double x = A[(size_t)r + (size_t)col * lda];
+      ss += x * x;
+      mx = fmax(mx, fabs(x));
+#pragma unroll
+      for (int q = 0; q < NB - 1; q++)
+        if (q < nq) d[q] += x * A[(size_t)r + (size_t)(col + 1 + q) * lda];
+    }
+    ss = wsum(ss);
+    mx = wmax(mx);
+#pragma unroll
+    for (int q = 0; q < NB - 1; q++) d[q] = wsum(d[q]);
+    if (lane == 0) {
+      s.red[wave][0] = ss;
+      s.red[wave][1] = mx;
+#pragma unroll
+      for (int q = 0; q < NB - 1; q++) s.red[wave][2 + q] = d[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double sst = 0.0, mxt = 0.0;
+      for (int w = 0; w < NW; w++) {
+        sst += s.red[w][0];
+        mxt = fmax(mxt, s.red[w][1]);
+      }
+      s.sc[3] = mxt;
+      s.sc[2] = sst;
+    }
+    __syncthreads();
+    double xnorm;
+    const double mxt = s.sc[3];
+    if (mxt != 0.0 && (mxt < 1e-150 || mxt > 1e150)) {
+      // sum of squares out of range: a scaled pass (the purpose of dnrm2)
+      const double inv = 1.0 / mxt;
+      double s2 = 0.0;
       for (int r = col + 1 + tid; r < m; r += BT) {
-        const double x = A[(size_t)r + (size_t)col * lda];
-        ss += x * x;
-        mx = fmax(mx, fabs(x));
-#pragma unroll
-        for (int q = 0; q < NB - 1; q++)
-          if (q < nq) d[q] += x * A[(size_t)r + (size_t)(col + 1 + q) * lda];
+        const double x = A[(size_t)r + (size_t)col * lda] * inv;
+        s2 += x * x;
       }
-      ss = wsum(ss);
-      mx = wmax(mx);
-#pragma unroll
-      for (int q = 0; q < NB - 1; q++) d[q] = wsum(d[q]);
-      if (lane == 0) {
-        s.red[wave][0] = ss;
-        s.red[wave][1] = mx;
-#pragma unroll
-        for (int q = 0; q < NB - 1; q++) s.red[wave][2 + q] = d[q];
-      }
+      s2 = wsum(s2);
       __syncthreads();
-      if (tid == 0) {
-        double sst = 0.0, mxt = 0.0;
-        for (int w = 0; w < 4; w++) {
-          sst += s.red[w][0];
-          mxt = fmax(mxt, s.red[w][1]);
-        }
-        s.sc[3] = mxt;
-        s.sc[2] = sst;
-      }
+      if (lane == 0) s.red[wave][0] = s2;
       __syncthreads();
-      double xnorm;
-      const double mxt = s.sc[3];
-      if (mxt != 0.0 && (mxt < 1e-150 || mxt > 1e150)) {
-        // sum of squares out of range: a scaled pass (LAPACK dnrm2's purpose)
-        const double inv = 1.0 / mxt;
-        double s2 = 0.0;
-        for (int r = col + 1 + tid; r < m; r += BT) {
-          const double x = A[(size_t)r + (size_t)col * lda] * inv;
-          s2 += x * x;
-        }
-        s2 = wsum(s2);
-        __syncthreads();
-        if (lane == 0) s.red[wave][0] = s2;
-        __syncthreads();
-        xnorm = mxt * sqrt(s.red[0][0] + s.red[1][0] + s.red[2][0] + s.red[3][0]);
-      } else {
-        xnorm = sqrt(s.sc[2]);
+      double t = 0.0;
+      for (int w = 0; w < NW; w++) t += s.red[w][0];
+      xnorm = mxt * sqrt(t);
+    } else {
+      xnorm = sqrt(s.sc[2]);
+    }
+    if (tid == 0) {
+      const double alpha = A[(size_t)col + (size_t)col * lda];
+      double beta = alpha, tau = 0.0, scale = 0.0;
+      if (xnorm != 0.0) {
+        const double aa = fabs(alpha);
+        const double w = fmax(aa, xnorm), z = fmin(aa, xnorm);
+        const double nrm = (z == 0.0) ? w : w * sqrt(1.0 + (z / w) * (z / w));
+        beta = -copysign(nrm, alpha);
+        tau = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
       }
-      if (tid == 0) {
-        const double alpha = A[(size_t)col + (size_t)col * lda];
-        double beta = alpha, tau = 0.0, scale = 0.0;
-        if (xnorm != 0.0) {
-          const double aa = fabs(alpha);
-          const double w = fmax(aa, xnorm), z = fmin(aa, xnorm);
-          const double nrm = (z == 0.0) ? w : w * sqrt(1.0 + (z / w) * (z / w));
-          beta = -copysign(nrm, alpha);
-          tau = (beta - alpha) / beta;
-          scale = 1.0 / (alpha - beta);
-        }
-        s.sc[0] = beta;
-        s.sc[1] = tau;
-        s.sc[2] = scale;
-        s.tau[c] = tau;
-        for (int q = 0; q < nq; q++) {
-          double dq = 0.0;
-          for (int w = 0; w < 4; w++) dq += s.red[w][2 + q];
-          s.w[q] = tau * (A[(size_t)col + (size_t)(col + 1 + q) * lda] + dq * scale);
+      s.sc[0] = beta;
+      s.sc[1] = tau;
+      s.sc[2] = scale;
+      s.tau[c] = tau;
+      for (int q = 0; q < nq; q++) {
+        double dq = 0.0;
+        for (int w = 0; w < NW; w++) dq += s.red[w][2 + q];
+        s.wv[q] = tau * (A[(size_t)col + (size_t)(col + 1 + q) * lda] + dq * scale);
+      }
+    }
+    __syncthreads();
+    const double beta = s.sc[0], tau = s.sc[1], scale = s.sc[2];
+    if (tau != 0.0) {
+      for (int r = col + tid; r < m; r += BT) {
+        if (r == col) {
+          A[(size_t)col + (size_t)col * lda] = beta;
+          for (int q = 0; q < nq; q++) A[(size_t)col + (size_t)(col + 1 + q) * lda] -= s.wv[q];
+        } else {
+          const double v = A[(size_t)r + (size_t)col * lda] * scale;
+          A[(size_t)r + (size_t)col * lda] = v;
+          for (int q = 0; q < nq; q++) A[(size_t)r + (size_t)(col + 1 + q) * lda] -= v * s.wv[q];
         }
       }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- one least-squares problem ----
+__device__ void solve_problem(const BProb &p, Smem &s) {
+  const int m = p.m, n = p.n, k = p.k;
+  double reg[PF];
+
+  // pass a: G1 of the first panel
+  {
+    THREAD_IDS;
+    const Pan P{0, min(NB, n), n - min(NB, n), n - min(NB, n) + k};
+    d4_t g = {0.0, 0.0, 0.0, 0.0};
+    const int nch = (m + RC - 1) / RC;
+    fetch(p, P, 0, false, reg);
+    for (int ch = 0; ch < nch; ch++) {
+      THREAD_IDS;
+      put(s, reg, false, P.jb);
       __syncthreads();
-      const double beta = s.sc[0], tau = s.sc[1], scale = s.sc[2];
-      if (tau != 0.0) {
-        for (int r = col + tid; r < m; r += BT) {
-          if (r == col) {
-            A[(size_t)col + (size_t)col * lda] = beta;
-            for (int q = 0; q < nq; q++) A[(size_t)col + (size_t)(col + 1 + q) * lda] -= s.w[q];
-          } else {
-            const double v = A[(size_t)r + (size_t)col * lda] * scale;
-            A[(size_t)r + (size_t)col * lda] = v;
-            for (int q = 0; q < nq; q++) A[(size_t)r + (size_t)(col + 1 + q) * lda] -= v * s.w[q];
-          }
-        }
-      }
+      if (ch + 1 < nch) fetch(p, P, (ch + 1) * RC, false, reg);
+      if (wave < 3) g = gram_tile(s, 0, wave, g);
       __syncthreads();
     }
-    // ---------------- 2. W = V^T [V, A trailing, B] ----------------
-    const int na = n - j0 - jb, ncf = jb + na + k;
-    const int nct = (ncf + 15) / 16;
-    d4_t acc[3];
+    if (wave < 3) gram_store(s.M0, wave, g);
+    __syncthreads();
+    pad_identity(s.M0, P.jb);
+    __syncthreads();
+  }
+
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    THREAD_IDS;
+    const int jb = min(NB, n - j0), na = n - j0 - jb;
+    const Pan P{j0, jb, na, na + k};
+    const int nch = (m - j0 + RC - 1) / RC;
+    const int nct = (P.nt + 15) / 16;  // column tiles of the trailing block
+
+    // ---- Cholesky-QR2 + reconstruction of the panel ----
+    if (wave == 0) {
+      const bool ok = small32::wave_chol32(&s.M0[0][0], s.D0, jb);
+      if (tid == 0) s.flag = ok ? 1 : 0;
+    }
+    __syncthreads();
+    bool cqr = s.flag != 0;
+    if (cqr) {
+      if (tid < PNB) inv_upper(s.M0, s.D0, s.MX, tid);
+      __syncthreads();
+      // pass b: Q1 = P R1^-1 (into chunk columns 32..63), G2 = Q1^T Q1, and
+      // the top block of Q1 into M3
+      d4_t g = {0.0, 0.0, 0.0, 0.0};
+      fetch(p, P, j0, false, reg);
+      for (int ch = 0; ch < nch; ch++) {
+        THREAD_IDS;
+        put(s, reg, false, jb);
+        __syncthreads();
+        if (ch + 1 < nch) fetch(p, P, j0 + (ch + 1) * RC, false, reg);
+        {
+          // RC / 16 x 2 tiles, one per wave
+          const int ri = wave % (RC / 16), cj = wave / (RC / 16);
+          d4_t q = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int u = 0; u < 3; u++) acc[u] = d4_t{0.0, 0.0, 0.0, 0.0};
-    for (int rb = j0; rb < m; rb += RC) {
-      // stage rows rb .. rb+RC-1 of all ncf columns (zero padded)
-      for (int e = tid; e < RC * nct * 16; e += BT) {
-        const int i = e % RC, q = e / RC, r = rb + i;
-        s.C[i][q] = (r < m && q < ncf) ? col_val(p, j0, jb, na, r, q) : 0.0;
+          for (int ks = 0; ks < NB / 4; ks++) {
+            const int c = 4 * ks + (lane >> 4);
+            q = mfma(s.C[ri * 16 + (lane & 15)][c], s.MX[c][cj * 16 + (lane & 15)], q);
+          }
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int r = ri * 16 + (lane >> 4) + 4 * v, c = cj * 16 + (lane & 15);
+            s.C[r][NB + c] = q[v];
+            if (ch == 0 && r < PNB) s.M3[r][c] = q[v];
+          }
+        }
+        __syncthreads();
+        if (wave >= 4 && wave < 7) g = gram_tile(s, NB, wave - 4, g);
+        __syncthreads();
+      }
+      if (wave >= 4 && wave < 7) gram_store(s.M2, wave - 4, g);
+      __syncthreads();
+      pad_identity(s.M2, jb);
+      __syncthreads();
+      if (wave == 0) {
+        const bool ok = small32::wave_chol32(&s.M2[0][0], s.D2, jb);
+        if (tid == 0) s.flag = ok ? 1 : 0;
+      }
+      __syncthreads();
+      cqr = s.flag != 0;
+    }
+    if (cqr) {
+      if (tid < PNB) inv_upper(s.M2, s.D2, s.TT, tid);  // R2^-1
+      __syncthreads();
+      // Q_top = Q1_top R2^-1, R^-1 = R1^-1 R2^-1, R = R2 R1
+      double qt[2], ri[2], rr[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int e = tid + h * BT, r = e / PNB, c = e % PNB;
+        double acc = 0.0;
+        for (int q = 0; q <= c; q++) acc += s.M3[r][q] * s.TT[q][c];
+        qt[h] = acc;
+        ri[h] = (r <= c) ? upper_prod(s.MX, s.TT, nullptr, r, c) : 0.0;
+        rr[h] = (r <= c) ? upper_prod(s.M2, s.M0, nullptr, r, c) : 0.0;
       }
       __syncthreads();
 #pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int e = tid + h * BT, r = e / PNB, c = e % PNB;
+        s.M3[r][c] = (r < jb && c < jb) ? qt[h] : (r == c ? 1.0 : 0.0);
+        s.MX[r][c] = ri[h];
+        s.M0[r][c] = rr[h];
+      }
+      __syncthreads();
+      if (wave == 0) {
+        const bool ok = small32::wave_lu_sign32(&s.M3[0][0], s.Sv, s.D3, jb);
+        if (tid == 0) s.flag = ok ? 1 : 0;
+      }
+      __syncthreads();
+      cqr = s.flag != 0;
+    }
+    if (cqr) {
+      if (tid < PNB) inv_upper(s.M3, s.D3, s.TT, tid);  // U^-1 (M3's upper part)
+      __syncthreads();
+      // transform of pass d: R^-1 S U^-1
+      double mf[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int e = tid + h * BT, r = e / PNB, c = e % PNB;
+        mf[h] = (r <= c) ? upper_prod(s.MX, s.TT, s.Sv, r, c) : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int e = tid + h * BT;
+        s.MX[e / PNB][e % PNB] = mf[h];
+      }
+      // T = -U V1^-T, row r by thread r: T V1^T = -U, right-looking over p
+      if (tid < PNB) {
+        const int r = tid;
+        double t[PNB];
+#pragma unroll
+        for (int c = 0; c < PNB; c++) t[c] = (c >= r) ? -s.M3[r][c] : 0.0;
+#pragma unroll
+        for (int q = 0; q < PNB; q++) {
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int c = q + 1; c < PNB; c++) t[c] -= t[q] * s.M3[c][q];
+        }
+#pragma unroll
+        for (int c = 0; c < PNB; c++) s.TT[r][c] = (r < jb && c < jb) ? t[c] : 0.0;
+      }
+      __syncthreads();
+    } else {
+      panel_columns(p.A, m, p.lda, j0, jb, s);
+    }
+
+    // ---- pass d: V (CQR: formed and written), W = V^T [V | trailing] ----
+    d4_t wacc[3];
+#pragma unroll
+    for (int u = 0; u < 3; u++) wacc[u] = d4_t{0.0, 0.0, 0.0, 0.0};
+    const int ntiles = 2 * (2 + nct);
+    fetch(p, P, j0, true, reg);
+    for (int ch = 0; ch < nch; ch++) {
+      THREAD_IDS;
+      const int rb = j0 + ch * RC;
+      put(s, reg, !cqr && ch == 0, jb);
+      __syncthreads();
+      if (ch + 1 < nch) fetch(p, P, rb + RC, true, reg);
+      if (cqr) {
+        // V = P (R^-1 S U^-1) on every row of the chunk (RC / 16 x 2 tiles,
+        // one per wave)
+        d4_t q4 = {0.0, 0.0, 0.0, 0.0};
+        const int ri = wave % (RC / 16), cj = wave / (RC / 16);
+#pragma unroll
+        for (int ks = 0; ks < NB / 4; ks++) {
+          const int c = 4 * ks + (lane >> 4);
+          q4 = mfma(s.C[ri * 16 + (lane & 15)][c], s.MX[c][cj * 16 + (lane & 15)], q4);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 4; v++) s.C[ri * 16 + (lane >> 4) + 4 * v][cj * 16 + (lane & 15)] = q4[v];
+        __syncthreads();
+        if (ch == 0) {
+          // the panel's top jb rows: V1 = L in the chunk; R_hh = S R on and
+          // above the diagonal and L below it into A
+          for (int e = tid; e < jb * NB; e += BT) {
+            const int i = e % jb, q = e / jb;
+            const double l = s.M3[i][q];
+            s.C[i][q] = (q < jb && i >= q) ? (i == q ? 1.0 : l) : 0.0;
+            if (q < jb)
+              p.A[(size_t)(j0 + i) + (size_t)(j0 + q) * p.lda] = (i <= q) ? s.Sv[i] * s.M0[i][q] : l;
+          }
+        }
+        store_panel(s, p, P, rb, ch == 0 ? jb : 0);
+        __syncthreads();
+      }
+#pragma unroll
       for (int u = 0; u < 3; u++) {
-        const int ct = wave + 4 * u;
-        if (ct < nct) {
+        const int id = wave + NW * u;
+        if (id < ntiles) {
+          const int ti = id & 1, tj = id >> 1;
 #pragma unroll
           for (int ks = 0; ks < RC / 4; ks++) {
             const int i = 4 * ks + (lane >> 4);
-            const double va = s.C[i][lane & 15];            // V^T(p = lane%16, row i)
-            const double cb = s.C[i][ct * 16 + (lane & 15)];  // C(row i, q)
-            acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(va, cb, acc[u], 0, 0, 0);
+            wacc[u] = mfma(s.C[i][ti * 16 + (lane & 15)], s.C[i][tj * 16 + (lane & 15)], wacc[u]);
           }
         }
       }
@@ -204,90 +531,138 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
     }
 #pragma unroll
     for (int u = 0; u < 3; u++) {
-      const int ct = wave + 4 * u;
-      if (ct < nct) {
+      const int id = wave + NW * u;
+      if (id < ntiles) {
+        const int ti = id & 1, tj = id >> 1;
 #pragma unroll
-        for (int v = 0; v < 4; v++) s.W[(lane >> 4) + 4 * v][ct * 16 + (lane & 15)] = acc[u][v];
+        for (int v = 0; v < 4; v++) s.W[ti * 16 + (lane >> 4) + 4 * v][tj * 16 + (lane & 15)] = wacc[u][v];
       }
     }
     __syncthreads();
-    // ---------------- 3. T from the Gram and tau; W2 = T^T W ----------------
-    if (tid == 0) {
-      for (int i = 0; i < jb; i++) {
-        const double ti = s.tau[i];
-        for (int r = 0; r < i; r++) {
-          double acc2 = 0.0;
-          if (ti != 0.0)
-            for (int q = r; q < i; q++) acc2 += s.T[r][q] * s.W[q][i];
-          s.T[r][i] = -ti * acc2;
+    if (!cqr) {
+      // dlarft('F','C') from the Gram V^T V (W's first 32 columns) and tau, one wave
+      if (wave == 0) {
+        for (int i = 0; i < PNB; i++) {
+          if (lane < PNB) {
+            const double ti = (i < jb) ? s.tau[i] : 0.0;
+            double acc = 0.0;
+            if (lane < i)
+              for (int q = lane; q < i; q++) acc += s.TT[lane][q] * s.W[q][i];
+            s.TT[lane][i] = (lane < i) ? -ti * acc : (lane == i ? ti : 0.0);
+          }
+          small32::wfence();
         }
-        s.T[i][i] = ti;
-        for (int r = i + 1; r < NB; r++) s.T[r][i] = 0.0;
       }
+      __syncthreads();
     }
-    __syncthreads();
-    const int nc = na + k;
-    // each thread owns one column of W: W2(:, q) = T^T W(:, q), rows >= jb zero
-    for (int q = jb + tid; q < jb + nc; q += BT) {
-      double wv[NB];
+    if (P.nt == 0) continue;  // nothing right of the last panel and no B
+
+    // ---- W2 = T^T W over the trailing columns ----
+    {
+      d4_t w2[2];
 #pragma unroll
-      for (int r = 0; r < NB; r++) wv[r] = s.W[r][q];
-#pragma unroll
-      for (int pr = 0; pr < NB; pr++) {
-        double a2 = 0.0;
-#pragma unroll
-        for (int r = 0; r <= pr; r++) a2 += s.T[r][pr] * wv[r];
-        s.W[pr][q] = (pr < jb) ? a2 : 0.0;
-      }
-    }
-    __syncthreads();
-    // ---------------- 4. [A trailing, B] -= V W2 ----------------
-    if (nc > 0) {
-      const int nct2 = (nc + 15) / 16;  // output column tiles over the A/B part
-      for (int rb = j0; rb < m; rb += RC) {
-        for (int e = tid; e < RC * (jb + nct2 * 16); e += BT) {
-          const int i = e % RC, q = e / RC, r = rb + i;
-          s.C[i][q] = (r < m && q < jb + nc) ? col_val(p, j0, jb, na, r, q) : 0.0;
-        }
-        __syncthreads();
-        // tiles: 2 row blocks x nct2 column tiles, 4 MFMAs each (K = 16)
-        for (int t = wave; t < 2 * nct2; t += 4) {
-          const int rblk = t & 1, ct = t >> 1;
-          d4_t dacc = d4_t{0.0, 0.0, 0.0, 0.0};
+      for (int u = 0; u < 2; u++) {
+        w2[u] = d4_t{0.0, 0.0, 0.0, 0.0};
+        const int id = wave + NW * u;
+        if (id < 2 * nct) {
+          const int ti = id & 1, tj = id >> 1;
 #pragma unroll
           for (int ks = 0; ks < NB / 4; ks++) {
-            const int pr = 4 * ks + (lane >> 4);
-            const double va = s.C[rblk * 16 + (lane & 15)][pr];     // V(row, p)
-            const double wb = s.W[pr][jb + ct * 16 + (lane & 15)];  // W2(p, q)
-            dacc = __builtin_amdgcn_mfma_f64_16x16x4f64(va, wb, dacc, 0, 0, 0);
+            const int q = 4 * ks + (lane >> 4);
+            w2[u] = mfma(s.TT[q][ti * 16 + (lane & 15)], s.W[q][NB + tj * 16 + (lane & 15)], w2[u]);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int id = wave + NW * u;
+        if (id < 2 * nct) {
+          const int ti = id & 1, tj = id >> 1;
+#pragma unroll
+          for (int v = 0; v < 4; v++) s.W[ti * 16 + (lane >> 4) + 4 * v][NB + tj * 16 + (lane & 15)] = w2[u][v];
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- pass U: trailing -= V W2; G1 of the next panel ----
+    const bool next = na > 0;
+    d4_t g = {0.0, 0.0, 0.0, 0.0};
+    fetch(p, P, j0, true, reg);
+    for (int ch = 0; ch < nch; ch++) {
+      THREAD_IDS;
+      const int rb = j0 + ch * RC;
+      put(s, reg, ch == 0, jb);
+      __syncthreads();
+      if (ch + 1 < nch) fetch(p, P, rb + RC, true, reg);
+#pragma unroll
+      for (int u = 0; u < (RC / 16) * (MAXT / 16) / NW + 1; u++) {
+        const int id = wave + NW * u;
+        if (id < (RC / 16) * nct) {
+          const int ri = id % (RC / 16), tj = id / (RC / 16);
+          d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int ks = 0; ks < NB / 4; ks++) {
+            const int q = 4 * ks + (lane >> 4);
+            acc = mfma(s.C[ri * 16 + (lane & 15)][q], s.W[q][NB + tj * 16 + (lane & 15)], acc);
           }
 #pragma unroll
-          for (int v = 0; v < 4; v++) s.C[rblk * 16 + (lane >> 4) + 4 * v][jb + ct * 16 + (lane & 15)] -= dacc[v];
+          for (int v = 0; v < 4; v++) s.C[ri * 16 + (lane >> 4) + 4 * v][NB + tj * 16 + (lane & 15)] -= acc[v];
         }
-        __syncthreads();
-        // write the updated rows back, column-contiguous
-        for (int e = tid; e < RC * nc; e += BT) {
-          const int i = e % RC, q = e / RC, r = rb + i;
-          if (r >= m) continue;
-          const double v = s.C[i][jb + q];
-          if (q < na) A[(size_t)r + (size_t)(j0 + jb + q) * lda] = v;
-          else p.B[(size_t)r + (size_t)(q - na) * p.ldb] = v;
-        }
-        __syncthreads();
       }
+      __syncthreads();
+      // rows from j0 + 32 on (the next panel's rows)
+      if (next && wave >= 5) g = gram_tile(s, NB, wave - 5, g, ch == 0 ? NB : 0);
+      store_trail(s, p, P, rb);
+      __syncthreads();
+    }
+    if (next) {
+      if (wave >= 5) gram_store(s.M0, wave - 5, g);
+      __syncthreads();
+      pad_identity(s.M0, min(NB, na));
+      __syncthreads();
     }
   }
-  // ---------------- back substitution R X = (Q^T B)(0:n) ----------------
-  for (int i = n - 1; i >= 0; i--) {
-    const double rii = A[(size_t)i + (size_t)i * lda];
-    if (tid < k) s.w[tid] = p.B[(size_t)i + (size_t)tid * p.ldb] / rii;
-    __syncthreads();
-    if (tid < k) p.B[(size_t)i + (size_t)tid * p.ldb] = s.w[tid];
-    for (int e = tid; e < i * k; e += BT) {
-      const int r = e % i, c = e / i;
-      p.B[(size_t)r + (size_t)c * p.ldb] -= A[(size_t)r + (size_t)i * lda] * s.w[c];
+
+  // ---- back substitution R X = (Q^T B)(0:n), blocks of 32 rows from the bottom ----
+  if (k > 0) {
+    for (int b0 = ((n - 1) / NB) * NB; b0 >= 0; b0 -= NB) {
+      THREAD_IDS;
+      const int bb = min(NB, n - b0);
+      for (int e = tid; e < PNB * PNB; e += BT) {
+        const int i = e % PNB, c = e / PNB;
+        s.M0[i][c] = (i < bb && c < bb) ? (i <= c ? p.A[(size_t)(b0 + i) + (size_t)(b0 + c) * p.lda] : 0.0)
+                                        : (i == c ? 1.0 : 0.0);
+        if (c < k) s.C[i][c] = (i < bb) ? p.B[(size_t)(b0 + i) + (size_t)c * p.ldb] : 0.0;
+      }
+      __syncthreads();
+      if (tid < k) {
+        double y[PNB];
+#pragma unroll
+        for (int i = 0; i < PNB; i++) y[i] = s.C[i][tid];
+#pragma unroll
+        for (int i = PNB - 1; i >= 0; i--) {
+          y[i] = y[i] / s.M0[i][i];
+#pragma unroll
+          for (int r = 0; r < i; r++) y[r] -= s.M0[r][i] * y[i];
+        }
+#pragma unroll
+        for (int i = 0; i < PNB; i++) s.C[i][tid] = y[i];
+      }
+      __syncthreads();
+      for (int e = tid; e < bb * k; e += BT) {
+        const int i = e % bb, c = e / bb;
+        p.B[(size_t)(b0 + i) + (size_t)c * p.ldb] = s.C[i][c];
+      }
+      for (int e = tid; e < b0 * k; e += BT) {
+        const int r = e % b0, c = e / b0;
+        double acc = 0.0;
+        for (int q = 0; q < bb; q++) acc += p.A[(size_t)r + (size_t)(b0 + q) * p.lda] * s.C[q][c];
+        p.B[(size_t)r + (size_t)c * p.ldb] -= acc;
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 

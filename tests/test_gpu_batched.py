@@ -103,6 +103,23 @@ def test_batched_device_pointers(ora, lila):
     say("worst factor error / tol", check_against_oracle(ora, b, out))
 
 
+def test_batched_graded_columns_take_the_column_loop(ora, lila):
+    """columns scaled over 8 orders of magnitude: the panels' Gram matrices
+    fail the Cholesky acceptance test (diagonal of R spread beyond 1e5), so
+    these panels are factored column by column with T from dlarft"""
+    import torch
+    sizes = [(300, 40, 3, 300, 300), (700, 128, 16, 701, 700), (96, 70, 1, 96, 96), (2000, 33, 2, 2000, 2000)]
+    b = Batch(sizes, 41)
+    for p, (m, n, k, lda, ldb) in enumerate(sizes):
+        A, _ = b.views(b.host0, p)
+        A[:, :] *= np.logspace(0, -8, n)[None, :]
+    b.buf.copy_(torch.from_numpy(b.host0))
+    say("batched graded", len(sizes), "problems")
+    assert lila.call("lila_dgels_batched", *b.arrays()) == 0
+    assert lila.call("lila_sync") == 0
+    check_against_oracle(ora, b, b.buf.cpu().numpy())
+
+
 def test_batched_host_pointers(ora, lila):
     sizes = make_sizes(21, 12, mmax=300) + EDGE[:2]
     b = Batch(sizes, 22)
