@@ -97,6 +97,15 @@ __device__ __forceinline__ int otid() {
   (void)lane;                                        \
   (void)wave
 
+// workgroup barrier ordering LDS only: the prefetch loads of the next chunk
+// stay in flight across it (__syncthreads() would wait for every
+// outstanding global access, prefetches included)
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ d4_t mfma(double a, double b, d4_t c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -175,17 +184,47 @@ __device__ __forceinline__ void store_panel(const Smem &s, const BProb &p, const
   }
 }
 
+// Stream chunks r0, r0 + RC, ... (nch of them) through LDS: two chunks are
+// prefetched into registers while body(ch) works on the one in LDS.  vmask0:
+// the first chunk's panel columns take V1's structure (see put).
+template <typename Body>
+__device__ __forceinline__ void stream(Smem &s, const BProb &p, const Pan &P, int r0, int nch,
+                                       bool trail, bool vmask0, Body &&body) {
+  double ra[PF], rb[PF];
+  fetch(p, P, r0, trail, ra);
+  if (nch > 1) fetch(p, P, r0 + RC, trail, rb);
+  for (int ch = 0; ch < nch; ch += 2) {
+    put(s, ra, vmask0 && ch == 0, P.jb);
+    lds_barrier();
+    if (ch + 2 < nch) fetch(p, P, r0 + (ch + 2) * RC, trail, ra);
+    body(ch);
+    lds_barrier();
+    if (ch + 1 < nch) {
+      put(s, rb, false, P.jb);
+      lds_barrier();
+      if (ch + 3 < nch) fetch(p, P, r0 + (ch + 3) * RC, trail, rb);
+      body(ch + 1);
+      lds_barrier();
+    }
+  }
+  __syncthreads();  // the pass's global stores visible to the workgroup
+}
+
 // ---- 32 x 32 pieces ----
 // Gram tile tt (0: (0,0), 1: (0,1), 2: (1,1)) of X^T X, X = chunk columns
-// c0..c0+31, over the chunk rows from ilo (a multiple of 4) on
-__device__ __forceinline__ d4_t gram_tile(const Smem &s, int c0, int tt, d4_t acc, int ilo = 0) {
+// c0..c0+ncol-1 (zero beyond), over the chunk rows from ilo (a multiple of 4) on
+__device__ __forceinline__ d4_t gram_tile(const Smem &s, int c0, int tt, d4_t acc, int ilo = 0,
+                                          int ncol = NB) {
   const int lane = otid() & 63;
   const int ti = (tt == 2) ? 1 : 0, tj = (tt == 0) ? 0 : 1;
+  // columns from ncol on are left out (they hold other data)
+  const bool va = ti * 16 + (lane & 15) < ncol, vb = tj * 16 + (lane & 15) < ncol;
 #pragma unroll
   for (int ks = 0; ks < RC / 4; ks++) {
     const int i = 4 * ks + (lane >> 4);
     if (4 * ks >= ilo)
-      acc = mfma(s.C[i][c0 + ti * 16 + (lane & 15)], s.C[i][c0 + tj * 16 + (lane & 15)], acc);
+      acc = mfma(va ? s.C[i][c0 + ti * 16 + (lane & 15)] : 0.0,
+                 vb ? s.C[i][c0 + tj * 16 + (lane & 15)] : 0.0, acc);
   }
   return acc;
 }
@@ -329,23 +368,16 @@ __device__ __noinline__ void panel_columns(double *A, int m, int lda, int j0, in
 // ---- one least-squares problem ----
 __device__ void solve_problem(const BProb &p, Smem &s) {
   const int m = p.m, n = p.n, k = p.k;
-  double reg[PF];
-
   // pass a: G1 of the first panel
   {
     THREAD_IDS;
     const Pan P{0, min(NB, n), n - min(NB, n), n - min(NB, n) + k};
     d4_t g = {0.0, 0.0, 0.0, 0.0};
     const int nch = (m + RC - 1) / RC;
-    fetch(p, P, 0, false, reg);
-    for (int ch = 0; ch < nch; ch++) {
+    stream(s, p, P, 0, nch, false, false, [&](int ch) {
       THREAD_IDS;
-      put(s, reg, false, P.jb);
-      __syncthreads();
-      if (ch + 1 < nch) fetch(p, P, (ch + 1) * RC, false, reg);
       if (wave < 3) g = gram_tile(s, 0, wave, g);
-      __syncthreads();
-    }
+    });
     if (wave < 3) gram_store(s.M0, wave, g);
     __syncthreads();
     pad_identity(s.M0, P.jb);
@@ -372,12 +404,8 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
       // pass b: Q1 = P R1^-1 (into chunk columns 32..63), G2 = Q1^T Q1, and
       // the top block of Q1 into M3
       d4_t g = {0.0, 0.0, 0.0, 0.0};
-      fetch(p, P, j0, false, reg);
-      for (int ch = 0; ch < nch; ch++) {
+      stream(s, p, P, j0, nch, false, false, [&](int ch) {
         THREAD_IDS;
-        put(s, reg, false, jb);
-        __syncthreads();
-        if (ch + 1 < nch) fetch(p, P, j0 + (ch + 1) * RC, false, reg);
         {
           // RC / 16 x 2 tiles, one per wave
           const int ri = wave % (RC / 16), cj = wave / (RC / 16);
@@ -385,7 +413,8 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
 #pragma unroll
           for (int ks = 0; ks < NB / 4; ks++) {
             const int c = 4 * ks + (lane >> 4);
-            q = mfma(s.C[ri * 16 + (lane & 15)][c], s.MX[c][cj * 16 + (lane & 15)], q);
+            if (cj == 1 || ks < NB / 8)  // MX is upper triangular
+              q = mfma(s.C[ri * 16 + (lane & 15)][c], s.MX[c][cj * 16 + (lane & 15)], q);
           }
 #pragma unroll
           for (int v = 0; v < 4; v++) {
@@ -394,10 +423,9 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
             if (ch == 0 && r < PNB) s.M3[r][c] = q[v];
           }
         }
-        __syncthreads();
+        lds_barrier();
         if (wave >= 4 && wave < 7) g = gram_tile(s, NB, wave - 4, g);
-        __syncthreads();
-      }
+      });
       if (wave >= 4 && wave < 7) gram_store(s.M2, wave - 4, g);
       __syncthreads();
       pad_identity(s.M2, jb);
@@ -479,14 +507,12 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
     d4_t wacc[3];
 #pragma unroll
     for (int u = 0; u < 3; u++) wacc[u] = d4_t{0.0, 0.0, 0.0, 0.0};
-    const int ntiles = 2 * (2 + nct);
-    fetch(p, P, j0, true, reg);
-    for (int ch = 0; ch < nch; ch++) {
+    // W tiles (row tile ti, column tile tj); the Gram V^T V (tj < 2) only
+    // for the column-loop panel's dlarft
+    const int tj0 = cqr ? 2 : 0, ntiles = 2 * (2 + nct - tj0);
+    stream(s, p, P, j0, nch, true, !cqr, [&](int ch) {
       THREAD_IDS;
       const int rb = j0 + ch * RC;
-      put(s, reg, !cqr && ch == 0, jb);
-      __syncthreads();
-      if (ch + 1 < nch) fetch(p, P, rb + RC, true, reg);
       if (cqr) {
         // V = P (R^-1 S U^-1) on every row of the chunk (RC / 16 x 2 tiles,
         // one per wave)
@@ -495,12 +521,13 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
 #pragma unroll
         for (int ks = 0; ks < NB / 4; ks++) {
           const int c = 4 * ks + (lane >> 4);
-          q4 = mfma(s.C[ri * 16 + (lane & 15)][c], s.MX[c][cj * 16 + (lane & 15)], q4);
+          if (cj == 1 || ks < NB / 8)  // MX is upper triangular
+            q4 = mfma(s.C[ri * 16 + (lane & 15)][c], s.MX[c][cj * 16 + (lane & 15)], q4);
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int v = 0; v < 4; v++) s.C[ri * 16 + (lane >> 4) + 4 * v][cj * 16 + (lane & 15)] = q4[v];
-        __syncthreads();
+        lds_barrier();
         if (ch == 0) {
           // the panel's top jb rows: V1 = L in the chunk; R_hh = S R on and
           // above the diagonal and L below it into A
@@ -513,13 +540,13 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
           }
         }
         store_panel(s, p, P, rb, ch == 0 ? jb : 0);
-        __syncthreads();
+        lds_barrier();
       }
 #pragma unroll
       for (int u = 0; u < 3; u++) {
         const int id = wave + NW * u;
         if (id < ntiles) {
-          const int ti = id & 1, tj = id >> 1;
+          const int ti = id & 1, tj = tj0 + (id >> 1);
 #pragma unroll
           for (int ks = 0; ks < RC / 4; ks++) {
             const int i = 4 * ks + (lane >> 4);
@@ -527,13 +554,12 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
           }
         }
       }
-      __syncthreads();
-    }
+    });
 #pragma unroll
     for (int u = 0; u < 3; u++) {
       const int id = wave + NW * u;
       if (id < ntiles) {
-        const int ti = id & 1, tj = id >> 1;
+        const int ti = id & 1, tj = tj0 + (id >> 1);
 #pragma unroll
         for (int v = 0; v < 4; v++) s.W[ti * 16 + (lane >> 4) + 4 * v][tj * 16 + (lane & 15)] = wacc[u][v];
       }
@@ -588,14 +614,11 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
 
     // ---- pass U: trailing -= V W2; G1 of the next panel ----
     const bool next = na > 0;
+    const int jbn = min(NB, na);  // width of the next panel
     d4_t g = {0.0, 0.0, 0.0, 0.0};
-    fetch(p, P, j0, true, reg);
-    for (int ch = 0; ch < nch; ch++) {
+    stream(s, p, P, j0, nch, true, true, [&](int ch) {
       THREAD_IDS;
       const int rb = j0 + ch * RC;
-      put(s, reg, ch == 0, jb);
-      __syncthreads();
-      if (ch + 1 < nch) fetch(p, P, rb + RC, true, reg);
 #pragma unroll
       for (int u = 0; u < (RC / 16) * (MAXT / 16) / NW + 1; u++) {
         const int id = wave + NW * u;
@@ -611,16 +634,15 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
           for (int v = 0; v < 4; v++) s.C[ri * 16 + (lane >> 4) + 4 * v][NB + tj * 16 + (lane & 15)] -= acc[v];
         }
       }
-      __syncthreads();
-      // rows from j0 + 32 on (the next panel's rows)
-      if (next && wave >= 5) g = gram_tile(s, NB, wave - 5, g, ch == 0 ? NB : 0);
+      lds_barrier();
+      // rows from j0 + 32 on (the next panel's rows) and its jbn columns
+      if (next && wave >= 5) g = gram_tile(s, NB, wave - 5, g, ch == 0 ? NB : 0, jbn);
       store_trail(s, p, P, rb);
-      __syncthreads();
-    }
+    });
     if (next) {
       if (wave >= 5) gram_store(s.M0, wave - 5, g);
       __syncthreads();
-      pad_identity(s.M0, min(NB, na));
+      pad_identity(s.M0, jbn);
       __syncthreads();
     }
   }
