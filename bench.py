@@ -15,8 +15,17 @@ Besides the JSON line's contract fields it reports:
                   library stream around each of its launches: achieved = its
                   algorithmic flops / its summed launch time, against FP64
                   MFMA peak (78.6 TFLOP/s);
+                  traffic is the class's HBM bytes per launch from the
+                  committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+                  (profiles/*_traffic.md), null when none matches;
   kernel_classes -- per-class launches, flops, bytes and event time of the
                   first warm-up step;
+  mixed        -- the mixed-precision form (lila_param[0] = 1) on the same
+                  input, and the per-kappa accuracy of both forms;
+  batch        -- 4096 uneven least-squares problems (m ~ U{64..4096},
+                  n ~ U{8..min(128,m)}, nrhs ~ U{1..16}) solved by one
+                  persistent launch replayed from a hipGraph: QR/s, and the
+                  oracle's per-problem dgels on the host cores beside it;
   cpu_baseline -- the CPU oracle (a C restatement of LILA's w03 recursion,
                   BLAS-3 steps through the host OpenBLAS, all its threads) on
                   a bounded sample of the same workload, rank 0 only;
@@ -133,6 +142,23 @@ def mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, steps, bwd_p
     del U, V, G, Ak, UV
     torch.cuda.empty_cache()
     return out
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summaries (profiles/*_traffic.md: FETCH_SIZE and WRITE_SIZE runs scaled
+    by a copy of known size), or None"""
+    import glob
+    for path in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.md")):
+        name, val = None, None
+        for line in open(path):
+            if line.startswith("kernel: "):
+                name = line[len("kernel: "):].strip()
+            elif line.startswith("traffic_bytes_per_launch: "):
+                val = float(line.split(":", 1)[1])
+        if name == kernel and val is not None:
+            return val
+    return None
 
 
 def read_prof(lib):
@@ -464,13 +490,15 @@ def main():
         if bound == "mfma":
             ach = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
             roof = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK, "unit": "TFLOP/s",
-                    "frac": ach / FP64_MFMA_PEAK, "traffic": None, "kernel": dom["name"],
+                    "frac": ach / FP64_MFMA_PEAK, "traffic": measured_traffic(dom["name"]),
+                    "kernel": dom["name"],
                     "launches": dom["launches"], "avg_launch_ms": dom["ms"] / dom["launches"],
                     "share_of_step": dom["ms"] / (dt * 1e3 * args.steps)}
         else:
             ach = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
             roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s",
-                    "frac": ach / HBM_PEAK, "traffic": None, "kernel": dom["name"],
+                    "frac": ach / HBM_PEAK, "traffic": measured_traffic(dom["name"]),
+                    "kernel": dom["name"],
                     "launches": dom["launches"], "avg_launch_ms": dom["ms"] / dom["launches"],
                     "share_of_step": dom["ms"] / (dt * 1e3 * args.steps)}
 
