@@ -243,21 +243,6 @@ __device__ __forceinline__ void pad_identity(double (*M)[LDM], int jb) {
   if (tid < PNB && tid >= jb) M[tid][tid] = 1.0;
 }
 
-// X = R^-1 for upper R with D = 1 / diag(R): thread c < 32 forms column c
-__device__ __forceinline__ void inv_upper(const double (*R)[LDM], const double *D,
-                                          double (*X)[LDM], int c) {
-  double x[PNB];
-#pragma unroll
-  for (int i = PNB - 1; i >= 0; i--) {
-    double acc = 0.0;
-#pragma unroll
-    for (int q = i + 1; q < PNB; q++) acc += R[i][q] * x[q];
-    x[i] = (i < c) ? -D[i] * acc : (i == c ? D[c] : 0.0);
-  }
-#pragma unroll
-  for (int i = 0; i < PNB; i++) X[i][c] = x[i];
-}
-
 // (X Y)(r, c) for upper X, Y; colscale multiplies X's column q by sc[q]
 __device__ __forceinline__ double upper_prod(const double (*X)[LDM], const double (*Y)[LDM],
                                              const double *sc, int r, int c) {
@@ -399,7 +384,7 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
     __syncthreads();
     bool cqr = s.flag != 0;
     if (cqr) {
-      if (tid < PNB) inv_upper(s.M0, s.D0, s.MX, tid);
+      if (wave == 0) small32::inv_upper32(&s.M0[0][0], s.D0, &s.MX[0][0]);
       __syncthreads();
       // pass b: Q1 = P R1^-1 (into chunk columns 32..63), G2 = Q1^T Q1, and
       // the top block of Q1 into M3
@@ -438,7 +423,7 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
       cqr = s.flag != 0;
     }
     if (cqr) {
-      if (tid < PNB) inv_upper(s.M2, s.D2, s.TT, tid);  // R2^-1
+      if (wave == 0) small32::inv_upper32(&s.M2[0][0], s.D2, &s.TT[0][0]);  // R2^-1
       __syncthreads();
       // Q_top = Q1_top R2^-1, R^-1 = R1^-1 R2^-1, R = R2 R1
       double qt[2], ri[2], rr[2];
@@ -468,7 +453,7 @@ __device__ void solve_problem(const BProb &p, Smem &s) {
       cqr = s.flag != 0;
     }
     if (cqr) {
-      if (tid < PNB) inv_upper(s.M3, s.D3, s.TT, tid);  // U^-1 (M3's upper part)
+      if (wave == 0) small32::inv_upper32(&s.M3[0][0], s.D3, &s.TT[0][0]);  // U^-1 (M3's upper part)
       __syncthreads();
       // transform of pass d: R^-1 S U^-1
       double mf[2];

@@ -68,7 +68,7 @@ constexpr int OFF_PUB = OFF_G2 + PNB * PNB;  // U^-1 (PNB*PNB), S (PNB), flag
 constexpr int SCRATCH = OFF_PUB + PNB * PNB + PNB + 8;
 // LDS past the slab (doubles): the larger of the two kernels' needs
 constexpr int HH_TAIL = 3 * PNB * PNB + 64 + 128 + 3 * PNB + 16 + SLOT;
-constexpr int CQR_TAIL = 3 * PNB * LDM + 3 * PNB + 2;
+constexpr int CQR_TAIL = 4 * PNB * LDM + 3 * PNB + 2;
 constexpr int LDS_TAIL = (HH_TAIL > CQR_TAIL ? HH_TAIL : CQR_TAIL) + 2;
 // dynamic LDS budget: 160 KiB minus the kernels' static LDS
 constexpr int LDS_DYN_BYTES = 163840 - 16;
@@ -507,67 +507,43 @@ __device__ void slab_gram(const double *X, int RS, int nloc, int nb, double *M) 
 
 // X (nloc rows of the slab) <- (X diag(s)) R^-1 for upper R (32 x 32, ld
 // LDM, identity beyond nb) with D = 1 / diag(R); s == nullptr means no
-// scaling.  Two rows per thread held in registers, right-looking over the
-// columns: y_c *= D_c, then y_q -= y_c R(c,q) for q > c, R's row c read from
-// LDS as 16-byte broadcast pairs.  The scheduling barrier keeps each step's
-// loads inside the step (hoisted together they would not fit in
-// registers).
-__device__ __forceinline__ void rows_solve_upper(double *X, int RS, int nloc, int nb,
-                                              const double *R, const double *D,
-                                              const double *s, double *dummy) {
-  for (int base = 0; base < nloc; base += 2 * PNT) {
-    // rows past nloc compute on a copy of row nloc - 1 and store nothing
-    const int i0 = base + (int)threadIdx.x, i1 = i0 + PNT;
-    const bool w0 = i0 < nloc, w1 = i1 < nloc;
-    const int lr0 = min(i0, nloc - 1), lr1 = min(i1, nloc - 1);
-    double y0[PNB], y1[PNB];
+// scaling.  R^-1 is formed in MI by one wave, then every wave multiplies
+// 16-row tiles of the slab by diag(s) R^-1 on v_mfma_f64_16x16x4f64 (R^-1
+// upper: the first output column block takes half the K steps) and writes
+// them back in place (a wave reads all of its tile's rows before writing).
+__device__ __forceinline__ void slab_mul(double *X, int RS, int nloc, int nb, const double *s,
+                                         const double *MI) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ntile = (nloc + 15) / 16, cl = lane & 15;
+  for (int t = wave; t < ntile; t += PNT / 64) {
+    const int row = 16 * t + cl;
+    d4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < PNB; c++) {
-      // clamped, unconditional loads and selects: a branch per column makes
-      // the register allocator spill the rows
-      const int cc = min(c, nb - 1);
-      const double sc = s ? s[cc] : 1.0;
-      const double v0 = X[lr0 + cc * RS] * sc, v1 = X[lr1 + cc * RS] * sc;
-      y0[c] = (c < nb) ? v0 : 0.0;
-      y1[c] = (c < nb) ? v1 : 0.0;
+    for (int ks = 0; ks < PNB / 4; ks++) {
+      const int k = 4 * ks + (lane >> 4);
+      const double a = (row < nloc && k < nb) ? X[row + k * RS] : 0.0;
+      const double sk = s ? s[k] : 1.0;
+      if (ks < PNB / 8)
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sk * MI[k * LDM + cl], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sk * MI[k * LDM + 16 + cl], acc1, 0, 0, 0);
     }
 #pragma unroll
-    for (int c = 0; c < PNB; c++) {
-      __builtin_amdgcn_sched_barrier(0);
-      const double dc = D[c];
-      y0[c] *= dc;
-      y1[c] *= dc;
-      const double *Rc = R + c * LDM;
-      int q = c + 1;
-      if (q & 1) {
-        if (q < PNB) {
-          const double r = Rc[q];
-          y0[q] -= y0[c] * r;
-          y1[q] -= y1[c] * r;
-        }
-        q++;
+    for (int v = 0; v < 4; v++) {
+      const int r = 16 * t + (lane >> 4) + 4 * v;
+      if (r < nloc) {
+        if (cl < nb) X[r + cl * RS] = acc0[v];
+        if (16 + cl < nb) X[r + (16 + cl) * RS] = acc1[v];
       }
-#pragma unroll
-      for (; q < PNB; q += 2) {
-        const double2 r = *reinterpret_cast<const double2 *>(Rc + q);
-        y0[q] -= y0[c] * r.x;
-        y0[q + 1] -= y0[c] * r.y;
-        y1[q] -= y1[c] * r.x;
-        y1[q + 1] -= y1[c] * r.y;
-      }
-    }
-    // columns >= nb hold 0 and are stored to column nb - 1 first, so the
-    // store of y[nb - 1] comes last and wins; a row past nloc stores to a
-    // dummy LDS slot instead of a branch (a branch here spills)
-    double *d0 = w0 ? X + lr0 : dummy;
-    double *d1 = w1 ? X + lr1 : dummy;
-    const int st0 = w0 ? RS : 0, st1 = w1 ? RS : 0;
-#pragma unroll
-    for (int c = PNB - 1; c >= 0; c--) {
-      d1[min(c, nb - 1) * st1] = y1[c];
-      d0[min(c, nb - 1) * st0] = y0[c];
     }
   }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void slab_mul_inv(double *X, int RS, int nloc, int nb, const double *R,
+                                             const double *D, const double *s, double *MI) {
+  if (threadIdx.x < 64) small32::inv_upper32(R, D, MI);
+  __syncthreads();
+  slab_mul(X, RS, nloc, nb, s, MI);
 }
 
 // the fallback path out of line, so that its registers do not weigh on the
@@ -586,11 +562,11 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   double *X = smem;
   double *tail = smem + slab_doubles(nb, RS);
   double *M0 = tail, *M2 = M0 + PNB * LDM, *M3 = M2 + PNB * LDM;
-  double *Sv = M3 + PNB * LDM;  // 32 signs
+  double *M4 = M3 + PNB * LDM;  // inverse of the current triangular factor
+  double *Sv = M4 + PNB * LDM;  // 32 signs
   double *D0 = Sv + PNB;        // 1 / diag of the current triangular factor
   double *D2 = D0 + PNB;
   int *flag = (int *)(D2 + PNB);  // [0] proceed
-  double *dummy = D2 + PNB + 1;   // target of discarded stores
   double *G1 = a.scr + OFF_G1, *G2 = a.scr + OFF_G2, *pub = a.scr + OFF_PUB;
   unsigned nbar = 0;
 
@@ -619,7 +595,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   __syncthreads();
   if (flag[0]) {
-    rows_solve_upper(X, RS, nloc, nb, M0, D0, nullptr, dummy);
+    slab_mul_inv(X, RS, nloc, nb, M0, D0, nullptr, M4);
     __syncthreads();
     // ---- pass 2 ----
     slab_gram(X, RS, nloc, nb, M2);
@@ -645,7 +621,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     __syncthreads();
   }
   if (flag[0]) {
-    rows_solve_upper(X, RS, nloc, nb, M2, D2, nullptr, dummy);  // X = Q
+    slab_mul_inv(X, RS, nloc, nb, M2, D2, nullptr, M4);  // X = Q
     __syncthreads();
     if (g == 0) {
       // R = R2 R1 over R1 in M0 (upper), all threads, via registers
@@ -708,7 +684,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   if (a.Q) store_slab(X, RS, nb, nloc, a.Q + r0, a.ldq, Sv);
   __syncthreads();
   // V2 = (Q2 S) U^-1 (the top rows of workgroup 0 are replaced below)
-  rows_solve_upper(X, RS, nloc, nb, M2, D2, Sv, dummy);
+  slab_mul_inv(X, RS, nloc, nb, M2, D2, Sv, M4);
   __syncthreads();
   if (g == 0) {
     // top nb rows: R' = S R2 R1 on and above the diagonal, V1 = L below

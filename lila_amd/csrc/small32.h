@@ -133,6 +133,57 @@ __device__ __forceinline__ bool wave_lu_sign32(double *M, double *S, double *D, 
   return ok;
 }
 
+// X = R^-1 for upper R (ld LDM, identity beyond the used order; whatever R
+// holds below its diagonal is not read) with D = 1 / diag(R), by one whole
+// wave, in 16 x 16 blocks: R = [A B; 0 C] gives R^-1 = [A^-1, -A^-1 B C^-1;
+// 0, C^-1].  Lanes 0..15 form the columns of A^-1 and lanes 16..31 those of
+// C^-1 at the same time (back substitution, the column in registers, fully
+// unrolled: 16 sequential steps instead of 32); the off-diagonal block is two
+// 16 x 16 x 16 products on v_mfma_f64_16x16x4f64.
+__device__ __forceinline__ void inv_upper32(const double *R, const double *D, double *X) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  const int lane = opaque_lane(), c = lane & 15, b = (lane & 16);
+  if (lane < 32) {
+    double x[16];
+#pragma unroll
+    for (int i = 15; i >= 0; i--) {
+      double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+      for (int q = i + 1; q < 16; q++) {
+        const double t = R[(b + i) * LDM + b + q] * x[q];
+        if (q & 1) acc1 += t; else acc0 += t;
+      }
+      const double di = D[b + i];
+      x[i] = (i < c) ? -di * (acc0 + acc1) : (i == c ? di : 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      X[(b + i) * LDM + b + c] = x[i];
+      if (b == 0) X[(16 + i) * LDM + c] = 0.0;  // lower-left block
+    }
+  }
+  wfence();
+  // T = B C^-1 into the upper-right block, then X12 = -A^-1 T in place
+  d4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+    const int k = 4 * ks + (lane >> 4);
+    t = __builtin_amdgcn_mfma_f64_16x16x4f64(R[c * LDM + 16 + k], X[(16 + k) * LDM + 16 + c], t, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) X[((lane >> 4) + 4 * v) * LDM + 16 + c] = t[v];
+  wfence();
+  d4 u = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+    const int k = 4 * ks + (lane >> 4);
+    u = __builtin_amdgcn_mfma_f64_16x16x4f64(X[c * LDM + k], X[k * LDM + 16 + c], u, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) X[((lane >> 4) + 4 * v) * LDM + 16 + c] = -u[v];
+  wfence();
+}
+
 }  // namespace small32
 
 #endif
