@@ -110,6 +110,28 @@ __device__ __forceinline__ void max_coh(double *p, double v) {  // v >= 0
 // fences are needed: the arrival is issued only after this workgroup's
 // atomics completed, and the reads of the reduced values are issued only
 // after the poll saw every arrival.
+// grid_sync in two halves, so a workgroup can work between its arrival and
+// the wait (LDS-only work: its global writes before arrive are the ones
+// the others see)
+__device__ void grid_arrive(unsigned *cnt) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&cnt[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ void grid_wait(unsigned *cnt, unsigned target, int *err) {
+  if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(&cnt[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > (1u << 24)) {
+        atomicOr(err, 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 __device__ void grid_sync(unsigned *cnt, unsigned target, int *err) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -556,7 +578,7 @@ __device__ __noinline__ void hh_fallback(const PanelArgs &a, double *slab, doubl
 __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) void panel_cqr_kernel(PanelArgs a) {
   extern __shared__ double smem[];
   const int nb = a.nb, RS = a.rs, G = a.G;
-  const int g = blockIdx.x, tid = threadIdx.x, wave = tid >> 6;
+  const int g = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int r0 = g * a.rp;
   const int nloc = max(0, min(a.ml, r0 + a.rp) - r0);
   double *X = smem;
@@ -566,7 +588,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   double *Sv = M4 + PNB * LDM;  // 32 signs
   double *D0 = Sv + PNB;        // 1 / diag of the current triangular factor
   double *D2 = D0 + PNB;
-  int *flag = (int *)(D2 + PNB);  // [0] proceed
+  int *flag = (int *)(D2 + PNB);  // [0] proceed, [1] the LU's verdict
   double *G1 = a.scr + OFF_G1, *G2 = a.scr + OFF_G2, *pub = a.scr + OFF_PUB;
   unsigned nbar = 0;
 
@@ -624,22 +646,6 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     slab_mul_inv(X, RS, nloc, nb, M2, D2, nullptr, M4);  // X = Q
     __syncthreads();
     if (g == 0) {
-      // R = R2 R1 over R1 in M0 (upper), all threads, via registers
-      double racc[PNB * PNB / PNT];
-#pragma unroll
-      for (int u = 0; u < PNB * PNB / PNT; u++) {
-        const int t = tid + u * PNT, r = t / PNB, c = t % PNB;
-        double acc = 0.0;
-        if (r <= c)
-          for (int p = r; p <= c; p++) acc += M2[r * LDM + p] * M0[p * LDM + c];
-        racc[u] = acc;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < PNB * PNB / PNT; u++) {
-        const int t = tid + u * PNT, r = t / PNB, c = t % PNB;
-        M0[r * LDM + c] = racc[u];
-      }
       // top nb x nb of Q into M3 (identity beyond nb)
       for (int t = tid; t < PNB * PNB; t += PNT) {
         const int r = t / PNB, c = t % PNB;
@@ -648,18 +654,35 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       __syncthreads();
       if (wave == 0) {
         const bool ok = wave_lu_sign32(M3, Sv, D2, nb);
-        // publish U (upper part of M3, identity beyond nb), S and the flag
-        const int l = tid & 63;
-        for (int t = l; t < PNB * PNB; t += 64) {
-          const int r = t / PNB, c = t % PNB;
-          st_coh(&pub[r * PNB + c], r <= c ? M3[r * LDM + c] : 0.0);
-        }
-        if (l < nb) st_coh(&pub[PNB * PNB + l], Sv[l]);
-        if (l == 0) st_coh(&pub[PNB * PNB + PNB], ok ? 1.0 : 0.0);
+        if (tid == 0) flag[1] = ok ? 1 : 0;
       }
+      __syncthreads();
+      // publish U (upper part of M3, identity beyond nb), S and the flag
+      for (int t = tid; t < PNB * PNB; t += PNT) {
+        const int r = t / PNB, c = t % PNB;
+        st_coh(&pub[r * PNB + c], r <= c ? M3[r * LDM + c] : 0.0);
+      }
+      if (tid < nb) st_coh(&pub[PNB * PNB + tid], Sv[tid]);
+      if (tid == 0) st_coh(&pub[PNB * PNB + PNB], flag[1] ? 1.0 : 0.0);
     }
     nbar++;
-    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    grid_arrive(a.cnt);
+    if (g == 0) {
+      // while the others wait: R = R2 R1 (both upper) on MFMA, one 16 x 16
+      // tile per wave, into M0
+      const int ti = wave & 1, tj = wave >> 1;
+      d4_t racc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < PNB / 4; ks++) {
+        const int k = 4 * ks + (lane >> 4);
+        racc = __builtin_amdgcn_mfma_f64_16x16x4f64(M2[(ti * 16 + (lane & 15)) * LDM + k],
+                                                    M0[k * LDM + tj * 16 + (lane & 15)], racc, 0, 0, 0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < 4; v++) M0[(ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15)] = racc[v];
+    }
+    grid_wait(a.cnt, nbar * (unsigned)G, a.err);
     if (tid == 0) flag[0] = ld_coh(&pub[PNB * PNB + PNB]) != 0.0 ? 1 : 0;
     __syncthreads();
   }

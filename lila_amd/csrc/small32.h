@@ -37,22 +37,38 @@ __device__ __forceinline__ double rdlane(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// one wave: upper Cholesky of M (32 x 32, ld LDM, upper part read; rows
-// and columns >= nb hold the identity) in place, lower part zeroed, and the
-// reciprocals of R's diagonal into D.  Lane c keeps column c in registers;
-// step j scales row j by 1/R(j,j) (LAPACK dpotf2's scaling) and updates
-// A(r,c) -= R(j,r) R(j,c) with R(j,r) read from lane r.  Returns false if a
-// pivot is not positive or R's first nb diagonal entries spread by more
-// than 1e5.
-__device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb) {
-  const int lane = opaque_lane(), c = lane & 31;
-  double a[PNB];
+typedef double d4s_t __attribute__((ext_vector_type(4)));
+
+// one 16 x 16 x 16 product on v_mfma_f64_16x16x4f64 by one wave: returns the
+// lane's four entries of A B, A(i, k) at A[i * ars + k * acs], B(k, j) at
+// B[k * brs + j * bcs]; entry v of lane l is (l / 16 + 4 v, l % 16)
+__device__ __forceinline__ d4s_t mm16(const double *A, int ars, int acs, const double *B, int brs,
+                                      int bcs) {
+  const int lane = opaque_lane();
+  d4s_t acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int r = 0; r < PNB; r++) a[r] = M[r * LDM + c];
-  bool ok = true;
-  double dmax = 0.0, dmin = 1e308, myinv = 1.0;
+  for (int ks = 0; ks < 4; ks++) {
+    const int k = 4 * ks + (lane >> 4);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(lane & 15) * ars + k * acs],
+                                               B[k * brs + (lane & 15) * bcs], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// upper Cholesky of the 16 x 16 block of M at (o, o) in place (lower part of
+// the block zeroed), 1 / diag(R) into D[o..o+15]; lane c (c < 16) keeps
+// column c in registers, row broadcasts by v_readlane (all lanes run).
+// ok, dmin, dmax accumulate the pivot test and the spread of R's diagonal
+// over the first nb columns of the 32 x 32 matrix.
+__device__ __forceinline__ void chol16(double *M, double *D, int o, int nb, bool &ok, double &dmin,
+                                       double &dmax) {
+  const int lane = opaque_lane(), c = lane & 15;
+  double a[16];
 #pragma unroll
-  for (int j = 0; j < PNB; j++) {
+  for (int r = 0; r < 16; r++) a[r] = M[(o + r) * LDM + o + c];
+  double myinv = 1.0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
     const double d = rdlane(a[j], j);
     ok = ok && (d > 0.0);
     // 1 / sqrt(d) from the hardware estimate and two Newton steps
@@ -63,7 +79,7 @@ __device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb) {
     pinv = pinv * (1.5 - 0.5 * dd * pinv * pinv);
     pinv = pinv * (1.5 - 0.5 * dd * pinv * pinv);
     const double p = dd * pinv;
-    if (j < nb) {
+    if (o + j < nb) {
       dmax = fmax(dmax, p);
       dmin = fmin(dmin, p);
     }
@@ -71,37 +87,81 @@ __device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb) {
     const double rj = (c == j) ? p : a[j] * pinv;
     a[j] = rj;
 #pragma unroll
-    for (int r = j + 1; r < PNB; r++) a[r] -= rdlane(rj, r) * rj;
+    for (int r = j + 1; r < 16; r++) a[r] -= rdlane(rj, r) * rj;
   }
-  if (lane < PNB) {
+  if (lane < 16) {
 #pragma unroll
-    for (int r = 0; r < PNB; r++) M[r * LDM + c] = (r <= c) ? a[r] : 0.0;
-    D[c] = myinv;
+    for (int r = 0; r < 16; r++) M[(o + r) * LDM + o + c] = (r <= c) ? a[r] : 0.0;
+    D[o + c] = myinv;
   }
+  wfence();
+}
+
+// X (block origin, ld LDM) = inverse of the upper 16 x 16 block of R at
+// (o, o), D = 1 / diag; lanes 0..15 form the columns (back substitution,
+// column in registers)
+__device__ __forceinline__ void inv16(const double *R, const double *D, int o, double *X) {
+  const int lane = opaque_lane(), c = lane & 15;
+  double x[16];
+#pragma unroll
+  for (int i = 15; i >= 0; i--) {
+    double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+    for (int q = i + 1; q < 16; q++) {
+      const double t = R[(o + i) * LDM + o + q] * x[q];
+      if (q & 1) acc1 += t; else acc0 += t;
+    }
+    const double di = D[o + i];
+    x[i] = (i < c) ? -di * (acc0 + acc1) : (i == c ? di : 0.0);
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) X[i * LDM + c] = x[i];
+  }
+}
+
+// one wave: upper Cholesky of M (32 x 32, ld LDM, upper part read; rows
+// and columns >= nb hold the identity) in place, lower part zeroed, and the
+// reciprocals of R's diagonal into D, in 16 x 16 blocks: R11 = chol(A11),
+// R12 = R11^-T A12 (R11^-1 formed in the lower-left block, which is free,
+// then one MFMA product), A22 -= R12^T R12 (MFMA), R22 = chol(A22).  The
+// pivots are those of the column-by-column algorithm.  Returns false if a
+// pivot is not positive or R's first nb diagonal entries spread by more
+// than 1e5.
+__device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb) {
+  const int lane = opaque_lane();
+  bool ok = true;
+  double dmin = 1e308, dmax = 0.0;
+  chol16(M, D, 0, nb, ok, dmin, dmax);
+  inv16(M, D, 0, M + 16 * LDM);  // R11^-1 in the lower-left block
+  wfence();
+  const d4s_t r12 = mm16(M + 16 * LDM, 1, LDM, M + 16, LDM, 1);  // (R11^-1)^T A12
+#pragma unroll
+  for (int v = 0; v < 4; v++) M[((lane >> 4) + 4 * v) * LDM + 16 + (lane & 15)] = r12[v];
+  wfence();
+  const d4s_t s22 = mm16(M + 16, 1, LDM, M + 16, LDM, 1);  // R12^T R12
+#pragma unroll
+  for (int v = 0; v < 4; v++) M[(16 + (lane >> 4) + 4 * v) * LDM + 16 + (lane & 15)] -= s22[v];
+  wfence();
+  chol16(M, D, 16, nb, ok, dmin, dmax);
+#pragma unroll
+  for (int e = lane; e < 256; e += 64) M[(16 + e / 16) * LDM + e % 16] = 0.0;
   wfence();
   return ok && dmin > 1e-5 * dmax;
 }
 
-// one wave: sign-chosen LU of M (Q1 on entry, identity beyond nb) in place,
-// the column rule of lila-c/lila_dorgh2_3.c:8-21: at step k, S_k = -1 when
-// |1 - t| < |1 + t| for the diagonal t (column k negated), the pivot is
-// S_k t - 1, the column below is divided by it and the trailing block gets
-// the rank-1 update.  Lane c keeps column c in registers; the negation of
-// column k is carried as a sign (the steps before k are linear in column
-// k) and applied to its U entries at the end.  U (upper, diagonal the
-// pivots) and L (strict lower) overwrite M, S gets the signs and D the
-// reciprocal pivots.  Returns false when some |t| > 1 - 1e-10 (tau within
-// 1e-10 of 0 or 2).
-__device__ __forceinline__ bool wave_lu_sign32(double *M, double *S, double *D, int nb) {
-  const int lane = opaque_lane(), c = lane & 31;
-  double a[PNB];
+// sign-chosen LU (see wave_lu_sign32) of the 16 x 16 block of M at (o, o)
+// with the signs left unapplied: U above the diagonal as eliminated, the
+// pivots on it, L below; S[o..], D[o..] = signs, 1 / pivots
+__device__ __forceinline__ void lu16(double *M, double *S, double *D, int o, int nb, bool &ok) {
+  const int lane = opaque_lane(), c = lane & 15;
+  double a[16];
 #pragma unroll
-  for (int r = 0; r < PNB; r++) a[r] = M[r * LDM + c];
-  bool ok = true;
+  for (int r = 0; r < 16; r++) a[r] = M[(o + r) * LDM + o + c];
   double mys = 1.0, myinv = 1.0;
 #pragma unroll
-  for (int k = 0; k < PNB; k++) {
-    if (k < nb) {
+  for (int k = 0; k < 16; k++) {
+    if (o + k < nb) {
       const double t = rdlane(a[k], k);
       const bool flip = fabs(1.0 - t) < fabs(1.0 + t);
       if (fabs(t) > 1.0 - 1e-10) ok = false;
@@ -112,7 +172,7 @@ __device__ __forceinline__ bool wave_lu_sign32(double *M, double *S, double *D, 
       // U(k,c) for the trailing columns; finished columns c < k keep theirs
       const double uk = (c > k) ? a[k] : 0.0;
 #pragma unroll
-      for (int r = k + 1; r < PNB; r++) {
+      for (int r = k + 1; r < 16; r++) {
         const double l = rdlane(a[r], k) * f;  // L(r,k)
         a[r] = (c == k) ? l : a[r] - l * uk;
       }
@@ -123,11 +183,73 @@ __device__ __forceinline__ bool wave_lu_sign32(double *M, double *S, double *D, 
       }
     }
   }
-  if (lane < PNB) {
+  if (lane < 16) {
 #pragma unroll
-    for (int r = 0; r < PNB; r++) M[r * LDM + c] = (r < c) ? a[r] * mys : a[r];
-    S[c] = mys;
-    D[c] = myinv;
+    for (int r = 0; r < 16; r++) M[(o + r) * LDM + o + c] = a[r];
+    S[o + c] = mys;
+    D[o + c] = myinv;
+  }
+  wfence();
+}
+
+// one wave: sign-chosen LU of M (Q1 on entry, identity beyond nb) in place,
+// the column rule of lila-c/lila_dorgh2_3.c:8-21: at step k, S_k = -1 when
+// |1 - t| < |1 + t| for the diagonal t (column k negated), the pivot is
+// S_k t - 1, the column below is divided by it and the trailing block gets
+// the rank-1 update.  The negation of column k is carried as a sign (the
+// steps before k are linear in column k) and applied to its U entries at the
+// end.  In 16 x 16 blocks: LU of A11; L21 = A21 V^-1 with V = A11's U as
+// eliminated and diagonal pivot * sign (lanes 0..15, one row each) while
+// U12 = L11^-1 A12 (lanes 16..31, one column each); A22 -= L21 U12 (MFMA);
+// LU of A22 -- the same pivots and signs as the column-by-column order.  U
+// (upper, diagonal the pivots) and L (strict lower) overwrite M, S gets the
+// signs and D the reciprocal pivots.  Returns false when some |t| > 1 -
+// 1e-10 (tau within 1e-10 of 0 or 2).
+__device__ __forceinline__ bool wave_lu_sign32(double *M, double *S, double *D, int nb) {
+  const int lane = opaque_lane();
+  bool ok = true;
+  lu16(M, S, D, 0, nb, ok);
+  if (lane < 16) {
+    // row 16 + lane of L21: x V = A21 row, forward over the columns
+    double x[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) x[c] = M[(16 + lane) * LDM + c];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      double acc = x[c];
+#pragma unroll
+      for (int k = 0; k < c; k++) acc -= x[k] * M[k * LDM + c];
+      x[c] = acc * (S[c] * D[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 16; c++) M[(16 + lane) * LDM + c] = x[c];
+  } else if (lane < 32) {
+    // column 16 + j of U12 = L11^-1 A12 (L11 unit lower)
+    const int j = lane - 16;
+    double y[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) y[r] = M[r * LDM + 16 + j];
+#pragma unroll
+    for (int r = 1; r < 16; r++) {
+      double acc = y[r];
+#pragma unroll
+      for (int k = 0; k < r; k++) acc -= M[r * LDM + k] * y[k];
+      y[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) M[r * LDM + 16 + j] = y[r];
+  }
+  wfence();
+  const d4s_t s22 = mm16(M + 16 * LDM, LDM, 1, M + 16, LDM, 1);  // L21 U12
+#pragma unroll
+  for (int v = 0; v < 4; v++) M[(16 + (lane >> 4) + 4 * v) * LDM + 16 + (lane & 15)] -= s22[v];
+  wfence();
+  lu16(M, S, D, 16, nb, ok);
+  // the deferred signs: U(r, c) *= S[c] above the diagonal
+#pragma unroll
+  for (int e = lane; e < PNB * PNB; e += 64) {
+    const int r = e / PNB, c = e % PNB;
+    if (r < c) M[r * LDM + c] *= S[c];
   }
   wfence();
   return ok;
