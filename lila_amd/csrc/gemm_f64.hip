@@ -75,7 +75,22 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
     int kchunk, double *__restrict__ P) {
   constexpr int PA = 1, PB = 1;
   // SYRK form: a tile entirely below the diagonal writes nothing
-  if (c_upper && (int)(blockIdx.x * BM) > (int)(blockIdx.y * BN + BN - 1)) return;
+  // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+  // XCDs (each with its own L2); give XCD x the contiguous x-th eighth of the
+  // output tiles and walk it in 8 x 8 blocks, so the workgroups resident on
+  // one XCD share A row panels and B column panels through that L2
+  int tx = blockIdx.x, ty = blockIdx.y;
+  {
+    const int gx = gridDim.x, gy = gridDim.y, ntile = gx * gy;
+    if (ntile >= 512 && gx % 8 == 0 && gy % 8 == 0) {
+      const int lin = blockIdx.x + blockIdx.y * gx;
+      const int t = (lin % 8) * (ntile / 8) + lin / 8;
+      const int st = t / 64, w = t % 64, sx = gx / 8;
+      tx = (st % sx) * 8 + w % 8;
+      ty = (st / sx) * 8 + w / 8;
+    }
+  }
+  if (c_upper && tx * BM > ty * BN + BN - 1) return;
   constexpr int WMT = BM / 32;  // 16x16 tiles per wave along m
   constexpr int WNT = BN / 32;  // along n
   constexpr int NLA = BM * BK / NT;
@@ -90,7 +105,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int m0 = tx * BM, n0 = ty * BN;
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(K, kbeg + kchunk);
 
