@@ -8,16 +8,20 @@
 // sizes balance across the CUs.  A problem is factored in panels of 32
 // columns; the trailing columns of A and the columns of B are carried
 // together ("trailing" below).  Every pass streams the rows below the
-// panel's top in chunks of 32 rows through LDS (column-contiguous loads,
-// the next chunk prefetched into registers while the current one is used)
-// and does its arithmetic on v_mfma_f64_16x16x4f64:
+// panel's top in chunks of 64 rows through LDS (one wave reads 512
+// contiguous bytes of a column; two chunks prefetched into registers while
+// the current one is used; barriers inside a pass order LDS only, so the
+// prefetches stay in flight) and does its arithmetic on
+// v_mfma_f64_16x16x4f64:
 //   a. G1 = P^T P for the first panel P (later panels get G1 from pass U);
-//   b. Cholesky G1 = R1^T R1 (one wave), Q1 = P R1^-1 and G2 = Q1^T Q1;
+//   b. Cholesky G1 = R1^T R1 (one wave), Q1 = P R1^-1 (R1^-1 formed
+//      explicitly) and G2 = Q1^T Q1;
 //      Cholesky G2 = R2^T R2, R = R2 R1, the top block of Q = Q1 R2^-1;
 //      Householder reconstruction: sign-chosen LU Q_top S - I = L U,
 //      V1 = L, R_hh = S R, T = -U V1^-T;
 //   d. V2 = P2 (R^-1 S U^-1) written in place, and W = V^T [V | trailing]
-//      accumulated in the same pass; W2 = T^T W;
+//      accumulated in the same pass (V^T V too when the panel took the
+//      column loop); W2 = T^T W;
 //   U. trailing -= V W2, and G1 of the next panel accumulated from the
 //      updated columns in the same pass.
 // So each panel costs two reads of the panel and two reads plus one write
