@@ -70,6 +70,8 @@ constexpr int SCRATCH = OFF_PUB + PNB * PNB + PNB + 8;
 constexpr int HH_TAIL = 3 * PNB * PNB + 64 + 128 + 3 * PNB + 16 + SLOT;
 constexpr int CQR_TAIL = 4 * PNB * LDM + 3 * PNB + 2;
 constexpr int LDS_TAIL = (HH_TAIL > CQR_TAIL ? HH_TAIL : CQR_TAIL) + 2;
+// streamed panel (taller than the workgroups' LDS): five 32 x 32 matrices
+constexpr int STREAM_TAIL = 5 * PNB * LDM + 3 * PNB + 4;
 // dynamic LDS budget: 160 KiB minus the kernels' static LDS
 constexpr int LDS_DYN_BYTES = 163840 - 16;
 constexpr int LDS_SLAB_DOUBLES = LDS_DYN_BYTES / 8 - LDS_TAIL;
@@ -80,6 +82,7 @@ struct PanelArgs {
   double *T; int ldt;
   double *Q; int ldq;
   int G, rp, rs;      // workgroups, rows per slab, LDS column stride
+  int sr;             // streamed panel: rows per LDS sub-slab
   double *scr;        // persistent global scratch (SCRATCH doubles)
   unsigned *cnt;      // [0] arrivals, [1] exits
   int *err;
@@ -544,7 +547,7 @@ __device__ __forceinline__ void slab_mul(double *X, int RS, int nloc, int nb, co
     for (int ks = 0; ks < PNB / 4; ks++) {
       const int k = 4 * ks + (lane >> 4);
       const double a = (row < nloc && k < nb) ? X[row + k * RS] : 0.0;
-      const double sk = s ? s[k] : 1.0;
+      const double sk = (s && k < nb) ? s[k] : 1.0;  // s[k >= nb] is not set
       if (ks < PNB / 8)
         acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sk * MI[k * LDM + cl], acc0, 0, 0, 0);
       acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sk * MI[k * LDM + 16 + cl], acc1, 0, 0, 0);
@@ -742,6 +745,196 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   exit_protocol(a);
 }
 
+// ---------------------------------------------------------------------------
+// Cholesky-QR2 + Householder reconstruction for a panel taller than the
+// workgroups' LDS can hold: the same algorithm as panel_cqr_kernel, with each
+// workgroup's rows streamed through LDS in sub-slabs of a.sr rows once per
+// pass instead of staying resident -- Gram of A; Gram of A R1^-1; then
+// Q S = A (R^-1 S) and V = (Q S) U^-1 written out (R^-1 = R1^-1 R2^-1 and
+// U^-1 formed explicitly).  The panel is only read until the last pass, so
+// a failed acceptance test falls back to the column-by-column loop on the
+// untouched panel in global memory.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) void panel_cqr_stream_kernel(PanelArgs a) {
+  extern __shared__ double smem[];
+  const int nb = a.nb, RS = a.rs, G = a.G, SR = a.sr;
+  const int g = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int r0 = g * a.rp;
+  const int nloc = max(0, min(a.ml, r0 + a.rp) - r0);
+  double *X = smem;
+  double *tail = smem + slab_doubles(nb, RS);
+  double *M0 = tail, *M2 = M0 + PNB * LDM, *M3 = M2 + PNB * LDM;
+  double *M4 = M3 + PNB * LDM, *M5 = M4 + PNB * LDM;
+  double *Sv = M5 + PNB * LDM, *D0 = Sv + PNB, *D2 = D0 + PNB;
+  int *flag = (int *)(D2 + PNB);  // [0] proceed, [1] the LU's verdict
+  double *G1 = a.scr + OFF_G1, *G2 = a.scr + OFF_G2, *pub = a.scr + OFF_PUB;
+  unsigned nbar = 0;
+
+  // Gram of the sub-slabs (transformed by MI when given) into Gd (global)
+  auto gram_pass = [&](double *Gd, const double *MI, double *M) {
+    for (int s0 = 0; s0 < nloc; s0 += SR) {
+      const int n = min(SR, nloc - s0);
+      load_slab(a, X, r0 + s0, n);
+      if (MI) slab_mul(X, RS, n, nb, nullptr, MI);
+      slab_gram(X, RS, n, nb, M);
+      __syncthreads();
+      for (int t = tid; t < nb * nb; t += PNT) {
+        const int r = t / nb, c = t % nb;
+        if (r <= c) add_coh(&Gd[r * PNB + c], M[r * LDM + c]);
+      }
+      __syncthreads();
+    }
+  };
+  // Gram from global into M (identity beyond nb) and its Cholesky
+  auto chol_of = [&](const double *Gd, double *M, double *D) {
+    for (int t = tid; t < PNB * PNB; t += PNT) {
+      const int r = t / PNB, c = t % PNB;
+      double v = 0.0;
+      if (r < nb && c < nb) { if (r <= c) v = ld_coh(&Gd[r * PNB + c]); }
+      else if (r == c) v = 1.0;
+      M[r * LDM + c] = v;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const bool ok = wave_chol32(M, D, nb);
+      if (tid == 0) flag[0] = ok ? 1 : 0;
+    }
+    __syncthreads();
+  };
+  // upper X Y on MFMA into Z (one 16 x 16 tile per wave; Z may be X or Y)
+  auto upper_mul = [&](const double *Xm, const double *Ym, double *Z) {
+    const int ti = wave & 1, tj = wave >> 1;
+    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < PNB / 4; ks++) {
+      const int k = 4 * ks + (lane >> 4);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xm[(ti * 16 + (lane & 15)) * LDM + k],
+                                                 Ym[k * LDM + tj * 16 + (lane & 15)], acc, 0, 0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 4; v++) Z[(ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15)] = acc[v];
+    __syncthreads();
+  };
+
+  // ---- pass 1: G1 = A^T A, R1 ----
+  gram_pass(G1, nullptr, M0);
+  nbar++;
+  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  chol_of(G1, M0, D0);
+  if (flag[0]) {
+    // ---- pass 2: G2 = (A R1^-1)^T (A R1^-1), R2 ----
+    if (wave == 0) small32::inv_upper32(M0, D0, M4);
+    __syncthreads();
+    gram_pass(G2, M4, M2);
+    nbar++;
+    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    chol_of(G2, M2, D2);
+  }
+  if (flag[0]) {
+    // R^-1 = R1^-1 R2^-1 into M4
+    if (wave == 0) small32::inv_upper32(M2, D2, M5);
+    __syncthreads();
+    upper_mul(M4, M5, M4);
+    if (g == 0) {
+      // top nb x nb of Q = A R^-1 and its sign-chosen LU
+      load_slab(a, X, r0, min(nloc, SR));
+      slab_mul(X, RS, nb, nb, nullptr, M4);
+      for (int t = tid; t < PNB * PNB; t += PNT) {
+        const int r = t / PNB, c = t % PNB;
+        M3[r * LDM + c] = (r < nb && c < nb) ? X[r + c * RS] : (r == c ? 1.0 : 0.0);
+      }
+      __syncthreads();
+      if (wave == 0) {
+        const bool ok = wave_lu_sign32(M3, Sv, D2, nb);
+        if (tid == 0) flag[1] = ok ? 1 : 0;
+      }
+      __syncthreads();
+      for (int t = tid; t < PNB * PNB; t += PNT) {
+        const int r = t / PNB, c = t % PNB;
+        st_coh(&pub[r * PNB + c], r <= c ? M3[r * LDM + c] : 0.0);
+      }
+      if (tid < nb) st_coh(&pub[PNB * PNB + tid], Sv[tid]);
+      if (tid == 0) st_coh(&pub[PNB * PNB + PNB], flag[1] ? 1.0 : 0.0);
+    }
+    nbar++;
+    grid_arrive(a.cnt);
+    if (g == 0) upper_mul(M2, M0, M0);  // R = R2 R1 while the others wait
+    grid_wait(a.cnt, nbar * (unsigned)G, a.err);
+    if (tid == 0) flag[0] = ld_coh(&pub[PNB * PNB + PNB]) != 0.0 ? 1 : 0;
+    __syncthreads();
+  }
+
+  if (!flag[0]) {
+    // the panel is untouched: column-by-column Householder on it in global
+    // memory (every workgroup took the same decisions from the same data)
+    PanelArgs b = a;
+    b.rs = a.lda;
+    hh_factor<false, true>(b, a.A + r0, smem, r0, nloc, nbar);
+    exit_protocol(a);
+    return;
+  }
+
+  // ---- U, S; R^-1 S into M4, U^-1 into M5 ----
+  if (tid < nb) Sv[tid] = ld_coh(&pub[PNB * PNB + tid]);
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t / PNB, c = t % PNB;
+    M2[r * LDM + c] = ld_coh(&pub[r * PNB + c]);
+  }
+  __syncthreads();
+  if (tid < PNB) D2[tid] = 1.0 / M2[tid * LDM + tid];
+  if (tid < PNB && tid >= nb) Sv[tid] = 1.0;  // S beyond nb: 1
+  __syncthreads();
+  if (wave == 0) small32::inv_upper32(M2, D2, M5);
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t / PNB, c = t % PNB;
+    M4[r * LDM + c] *= Sv[c];
+  }
+  __syncthreads();
+  if (g == 0 && tid < nb) {
+    // T = -U V1^-T, one row per thread (as in panel_cqr_kernel)
+    const int r = tid;
+    double t[PNB];
+#pragma unroll
+    for (int c = 0; c < PNB; c++) {
+      const double u = -M2[r * LDM + c];
+      t[c] = (c >= r) ? u : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < PNB; q++) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int c = q + 1; c < PNB; c++) t[c] -= t[q] * M3[c * LDM + q];
+    }
+#pragma unroll
+    for (int c = PNB - 1; c >= 0; c--)
+      if (c >= r) a.T[(size_t)r + (size_t)min(c, nb - 1) * a.ldt] = t[c];
+  }
+  // ---- pass 3: Q S = A (R^-1 S), V = (Q S) U^-1 ----
+  for (int s0 = 0; s0 < nloc; s0 += SR) {
+    const int n = min(SR, nloc - s0);
+    load_slab(a, X, r0 + s0, n);
+    slab_mul(X, RS, n, nb, nullptr, M4);
+    if (a.Q) store_slab(X, RS, nb, n, a.Q + r0 + s0, a.ldq, nullptr);
+    __syncthreads();
+    slab_mul(X, RS, n, nb, nullptr, M5);
+    if (g == 0 && s0 == 0) {
+      // top nb rows: R' = S R2 R1 on and above the diagonal, V1 = L below
+      for (int t = tid; t < nb * nb; t += PNT) {
+        const int r = t % nb, c = t / nb;
+        X[r + c * RS] = (r <= c) ? Sv[r] * M0[r * LDM + c] : M3[r * LDM + c];
+      }
+      __syncthreads();
+    }
+    store_slab(X, RS, nb, n, a.A + r0 + s0, a.lda, nullptr);
+    __syncthreads();
+  }
+  exit_protocol(a);
+}
+
+size_t slab_doubles_h(int nb, int rs) { return (((size_t)nb * rs + 1) & ~(size_t)1); }
+static_assert(STREAM_TAIL > HH_TAIL, "the streamed panel's fallback uses the LDS base as its tail");
+
 size_t panel_smem_bytes(int nb, int rs) {
   return sizeof(double) * ((((size_t)nb * rs + 1) & ~(size_t)1) + LDS_TAIL);
 }
@@ -775,7 +968,12 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   // taller than 256 LDS slabs: the column-by-column kernel on the panel in
   // global memory, 256 workgroups
   const bool glob = rp > rmax;
-  const int rs = glob ? lda : (rp | 1);  // odd LDS column stride
+  // streamed Cholesky-QR2 panel: sub-slabs of the LDS that the fixed part
+  // leaves, odd column stride
+  const bool stream = glob && mode == 1;
+  int srs = (LDS_DYN_BYTES / 8 - STREAM_TAIL - 2) / nb;
+  if (!(srs & 1)) srs--;
+  const int rs = stream ? srs : glob ? lda : (rp | 1);  // odd LDS column stride
   // Co-residency of all G workgroups is what the in-kernel barrier needs.
   // It is checked once from the occupancy calculator (one workgroup per CU
   // at the largest LDS size, G <= CU count); a plain launch then keeps all
@@ -792,6 +990,8 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
     hipFuncSetAttribute((const void *)panel_cqr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)maxb);
     hipFuncSetAttribute((const void *)panel_hh_global_kernel,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
+    hipFuncSetAttribute((const void *)panel_cqr_stream_kernel,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
     int dev = 0, per_cu = 0, per_cu2 = 0;
     hipGetDevice(&dev);
@@ -810,13 +1010,17 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   PanelArgs pa;
   pa.ml = ml; pa.nb = nb; pa.A = A; pa.lda = lda; pa.T = T; pa.ldt = ldt;
   pa.Q = Q; pa.ldq = ldq; pa.G = G; pa.rp = rp; pa.rs = rs;
+  pa.sr = stream ? srs : rp;
   pa.scr = ld_panel_scratch();
   int *flags = ld_device_flags();
   pa.cnt = (unsigned *)(flags + 8);
   pa.err = flags;
   const void *fn;
   const char *cls;
-  if (glob) {
+  if (stream) {
+    fn = (const void *)panel_cqr_stream_kernel;
+    cls = "panel_cqr_stream_kernel";
+  } else if (glob) {
     fn = (const void *)panel_hh_global_kernel;
     cls = "panel_hh_global_kernel";
   } else if (G == 1) {
@@ -830,7 +1034,8 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
     cls = "panel_hh_kernel<false>";
   }
   void *args[] = {&pa};
-  const size_t shb = glob ? sizeof(double) * LDS_TAIL : panel_smem_bytes(nb, rs);
+  const size_t shb = stream ? sizeof(double) * (slab_doubles_h(nb, rs) + STREAM_TAIL)
+                   : glob ? sizeof(double) * LDS_TAIL : panel_smem_bytes(nb, rs);
   const int ph = ld_prof_begin(cls);
   hipError_t e;
   if (coop && G > 1) {

@@ -82,7 +82,7 @@ def test_recursive_both_modes(lila, ora, panel_mode, mode):
 
 def test_tall_recursive_beyond_lds_capacity(lila, ora, panel_mode):
     """m = 300000 puts every 32-column panel past 256 LDS slabs: the panels run
-    the column-by-column kernel on global memory"""
+    the streamed Cholesky-QR2 kernel (sub-slabs through LDS once per pass)"""
     m, n, ii, mt, nx = 300000, 96, 0, 64, 32
     say("tall recursive", m, n)
     panel_mode(1)
@@ -129,6 +129,35 @@ def test_cqr_falls_back_on_zero_subcolumns(lila, ora, panel_mode):
     assert np.array_equal(Ag, A)
     assert np.all(np.triu(Tg) == 0.0)
     assert np.array_equal(Ao, A)
+
+
+@pytest.mark.parametrize("m", [400000])
+def test_streamed_cqr_falls_back_on_ill_conditioned_tall_panel(lila, ora, panel_mode, m):
+    """the ill-conditioned panel above, past the LDS capacity: the streamed
+    kernel has only read the panel when its Cholesky test fails, and runs the
+    column-by-column loop on it in global memory"""
+    say("streamed cqr fallback: ill-conditioned", m)
+    panel_mode(1)
+    n = 24
+    base = ora.fill_normal(5, m, 1)
+    noise = ora.fill_normal(6, m, n)
+    A = np.asfortranarray(base @ np.ones((1, n)) + 1e-9 * noise)
+    (Ao, To), (Ag, Tg) = run_dgeqr3(lila, ora, A)
+    assert rel(np.triu(Ag[:n]), np.triu(Ao[:n])) < 1e-5
+
+
+@pytest.mark.parametrize("m", [400000])
+def test_streamed_cqr_falls_back_on_zero_subcolumns(lila, ora, panel_mode, m):
+    """upper-triangular tall panel: tau = 0 everywhere, the streamed kernel's
+    reconstruction test fails and the fallback returns the panel unchanged"""
+    say("streamed cqr fallback: tau = 0", m)
+    panel_mode(1)
+    n = 20
+    A = np.zeros((m, n), order="F")
+    A[:n] = np.triu(ora.fill_normal(9, n, n))
+    (Ao, To), (Ag, Tg) = run_dgeqr3(lila, ora, A)
+    assert np.array_equal(Ag, A)
+    assert np.all(np.triu(Tg) == 0.0)
 
 
 def test_panel_modes_agree_on_large_recursive_config1(lila, ora, panel_mode):
