@@ -203,6 +203,23 @@ int lila_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A, int ld
 int lila_dT2tau_w03(int m, int n, int i, int mt, double *T, int ldt, double *tau);
 int lila_dV2tau_w03(int m, int n, int i, int mt, double *A, int lda, double *tau);
 
+/* ---- the recursive QR library of qr3/src (its header qr3/src/qr2.h) ---- */
+
+/* qr3/src/qr2.h:20 (qr3/src/qr2_dgeqr3_R.c): recursive QR with R apart --
+ * V below A's diagonal, T (n x n upper), R (n x n upper).  A's diagonal and
+ * upper part are not an output (the reference leaves there each entry as it
+ * stood before its column's reflector; here they keep their input values);
+ * with R == A the result is dgeqr3's. */
+int qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+/* qr3/src/qr2.h:30 (qr3/src/qr2_dlarft3.c): T from the reflectors below A's
+ * diagonal and tau */
+int qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+/* qr3/src/qr2.h:25 (qr3/src/qr2_dorgqr3.c): explicit m x n Q from the
+ * reflectors below Q's diagonal and T, in place (T's diagonal equals tau) */
+int qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+/* qr3/src/qr2.h:27 (qr3/src/qr2_dorgqr3_VT2Q.c): the same Q */
+int qr2_dorgqr3_VT2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+
 /* ---- extensions (no counterpart in the reference header) ------------- */
 
 /* Status of the last call (0 ok, >0 numerical info, <0 error), after

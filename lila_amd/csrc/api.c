@@ -462,6 +462,41 @@ int lila_dgetrf_b(int m, int n, int i, int mt, double *A, int lda, double *Q, in
     return api_end();
 }
 
+/* ---- the recursive QR library of qr3/src (qr2.h) -------------------------- */
+
+int qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n);
+    const int alias = (R == A);
+    stage_t r = alias ? a : stage_in(R, ldr, n, n);
+    dv_qr2_dgeqr3_R(m, n, a.dev, lda, t.dev, ldt, alias ? a.dev : r.dev, alias ? lda : ldr);
+    stage_out(&a, 1); stage_out(&t, 1);
+    if (!alias) stage_out(&r, 1);
+    return api_end();
+}
+
+int qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n), u = stage_in(tau, n, n, 1);
+    dv_qr2_dlarft3(m, n, a.dev, lda, t.dev, ldt, u.dev);
+    stage_out(&a, 0); stage_out(&t, 1); stage_out(&u, 0);
+    return api_end();
+}
+
+int qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    BEGIN();
+    stage_t q = stage_in(Q, ldq, m, n), t = stage_in(T, ldt, n, n), u = stage_in(tau, n, n, 1);
+    dv_qr2_dorgqr3(m, n, q.dev, ldq, t.dev, ldt, u.dev);
+    stage_out(&q, 1); stage_out(&t, 0); stage_out(&u, 0);
+    return api_end();
+}
+
+/* qr3/src/qr2_dorgqr3_VT2Q.c: the same Q (T's diagonal blocks are copied
+ * into Q's upper part on the way and consumed there) */
+int qr2_dorgqr3_VT2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    return qr2_dorgqr3(m, n, Q, ldq, T, ldt, tau);
+}
+
 /* ---- least squares, generators, status --------------------------------- */
 
 int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb) {

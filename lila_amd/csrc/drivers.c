@@ -456,6 +456,50 @@ int dv_dgetrf_b(int m, int n, int i, int mt, double *A, int lda, double *Q, int 
     return 0;
 }
 
+/* ---- the recursive QR library of qr3/src (qr2.h) -------------------------- */
+
+/* qr3/src/qr2_dgeqr3_R.c:3-60: the recursive panel with R kept apart: V
+ * below A's diagonal, T (n x n upper), R (n x n upper).  A's diagonal and
+ * upper part are no output there (they end as each entry stood before its
+ * column's reflector); here they keep their input values.  With R == A the
+ * result is dgeqr3's. */
+int dv_qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr) {
+    if (n <= 0) return 0;
+    if (R == A && ldr == lda) return dv_dgeqr3(m, n, A, lda, T, ldt);
+    size_t mark = ld_ws_mark();
+    double *U = wsd((size_t)n * n);
+    ld_lacpy('U', n, n, A, lda, U, n);
+    dv_dgeqr3(m, n, A, lda, T, ldt);
+    ld_lacpy('U', n, n, A, lda, R, ldr);
+    ld_lacpy('U', n, n, U, n, A, lda);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_dlarft3.c:3-44: T (n x n upper, dlarft 'F','C') from the
+ * reflectors below A's diagonal and tau */
+int dv_qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    if (n <= 0) return 0;
+    ld_larft(m, n, A, lda, tau, T, ldt);
+    return 0;
+}
+
+/* qr3/src/qr2_dorgqr3.c:3-58: the explicit m x n Q = H_1 ... H_n [I; 0]
+ * over the reflectors stored below Q's diagonal, in place: Q = [I; 0] -
+ * V (T V1^T) with V copied out first.  T's diagonal stands for tau (the
+ * reference reads tau at its 1-column leaves, T elsewhere; they agree for
+ * any T of these reflectors). */
+int dv_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    (void)tau;
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *V = wsd((size_t)m * n);
+    ld_lacpy('L', m, n, Q, ldq, V, m);
+    panel_q(m, n, V, m, T, ldt, Q, ldq);
+    ld_ws_release(mark);
+    return 0;
+}
+
 /* least squares through (V, T, R): lila_dgels_qr3 semantic of
  * least-squares/main_ls_qr3.c:94-119.  B (m x k) overwritten, X in rows [0, n). */
 int dv_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb) {

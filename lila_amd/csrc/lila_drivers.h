@@ -93,4 +93,8 @@ int dq_dgeqrf_w03_recursive(const int *p, int m, int n, int i, int mt, int lwork
 int dq_dgeqrf_w03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, int m, int n, int i,
                          int mt);
 
+int dv_qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+int dv_qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int dv_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+
 #endif

@@ -1574,6 +1574,80 @@ int ora_dgels_hh(int m, int n, int k, double *A, int lda, double *B, int ldb, do
     return 0;
 }
 
+/* ---- the recursive QR library of qr3/src (qr2.h) ------------------------ */
+
+/* qr3/src/qr2_dgeqr3_R.c:3-60 (R apart from A; the W workspace is T12 when
+ * R aliases A, else R12) */
+int ora_qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr) {
+    if (n == 1) {
+        R[0] = A[0];
+        ora_dlarfg(m, R, A + 1, 1, T);
+        return 0;
+    }
+    int n1 = n / 2, n2 = n - n1, ldw;
+    double *A11 = A, *A12 = A + (size_t)n1 * lda, *A21 = A + n1, *A22 = A + n1 + (size_t)n1 * lda;
+    double *T11 = T, *T12 = T + (size_t)n1 * ldt, *T22 = T + n1 + (size_t)n1 * ldt;
+    double *R11 = R, *R12 = R + (size_t)n1 * ldr, *R22 = R + n1 + (size_t)n1 * ldr, *W;
+    ora_qr2_dgeqr3_R(m, n1, A11, lda, T11, ldt, R11, ldr);
+    if (A11 == R11) { W = T12; ldw = ldt; } else { W = R12; ldw = ldr; }
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) W[i + (size_t)j * ldw] = A12[(size_t)j * lda + i];
+    ora_dtrmm('L', 'L', 'T', 'U', n1, n2, 1.0, A11, lda, W, ldw);
+    ora_dgemm('T', 'N', n1, n2, m - n1, 1.0, A21, lda, A22, lda, 1.0, W, ldw);
+    ora_dtrmm('L', 'U', 'T', 'N', n1, n2, 1.0, T11, ldt, W, ldw);
+    ora_dgemm('N', 'N', m - n1, n2, n1, -1.0, A21, lda, W, ldw, 1.0, A22, lda);
+    ora_dtrmm('L', 'L', 'N', 'U', n1, n2, 1.0, A11, lda, W, ldw);
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) R12[(size_t)j * ldr + i] = A12[(size_t)j * lda + i] - W[(size_t)j * ldw + i];
+    ora_qr2_dgeqr3_R(m - n1, n2, A22, lda, T22, ldt, R22, ldr);
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) T12[i + (size_t)j * ldt] = A21[j + (size_t)i * lda];
+    ora_dtrmm('R', 'L', 'N', 'U', n1, n2, 1.0, A22, lda, T12, ldt);
+    ora_dgemm('T', 'N', n1, n2, m - n, 1.0, A21 + n2, lda, A22 + n2, lda, 1.0, T12, ldt);
+    ora_dtrmm('L', 'U', 'N', 'N', n1, n2, -1.0, T11, ldt, T12, ldt);
+    ora_dtrmm('R', 'U', 'N', 'N', n1, n2, 1.0, T22, ldt, T12, ldt);
+    return 0;
+}
+
+/* qr3/src/qr2_dlarft3.c:3-44 */
+int ora_qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    if (n <= 1) {
+        T[0] = tau[0];
+        return 0;
+    }
+    int n1 = n / 2, n2 = n - n1;
+    double *A21 = A + n1, *A31 = A + n, *A22 = A + n1 + (size_t)n1 * lda, *A32 = A + n + (size_t)n1 * lda;
+    double *T12 = T + (size_t)n1 * ldt, *T22 = T + n1 + (size_t)n1 * ldt;
+    ora_qr2_dlarft3(m, n1, A, lda, T, ldt, tau);
+    ora_qr2_dlarft3(m - n1, n2, A22, lda, T22, ldt, tau + n1);
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) T12[i + (size_t)j * ldt] = A21[j + (size_t)i * lda];
+    ora_dtrmm('R', 'L', 'N', 'U', n1, n2, 1.0, A22, lda, T12, ldt);
+    ora_dgemm('T', 'N', n1, n2, m - n, 1.0, A31, lda, A32, lda, 1.0, T12, ldt);
+    ora_dtrmm('L', 'U', 'N', 'N', n1, n2, -1.0, T, ldt, T12, ldt);
+    ora_dtrmm('R', 'U', 'N', 'N', n1, n2, 1.0, T22, ldt, T12, ldt);
+    return 0;
+}
+
+/* qr3/src/qr2_dorgqr3.c:3-58 */
+int ora_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    if (n <= 1) {
+        for (int i = 1; i < m; i++) Q[i] *= -tau[0];
+        Q[0] = 1.0 - tau[0];
+        return 0;
+    }
+    int n1 = n / 2, n2 = n - n1;
+    double *Q21 = Q + n1, *Q12 = Q + (size_t)n1 * ldq, *Q22 = Q + n1 + (size_t)n1 * ldq;
+    double *T22 = T + n1 + (size_t)n1 * ldt;
+    ora_qr2_dorgqr3(m - n1, n2, Q22, ldq, T22, ldt, tau + n1);
+    ora_dgemm('T', 'N', n1, n2, m - n1, 1.0, Q21, ldq, Q22, ldq, 0.0, Q12, ldq);
+    ora_dtrmm('L', 'U', 'N', 'N', n1, n2, 1.0, T, ldt, Q12, ldq);
+    ora_dgemm('N', 'N', m - n1, n2, n1, -1.0, Q21, ldq, Q12, ldq, 1.0, Q22, ldq);
+    ora_dtrmm('L', 'L', 'N', 'U', n1, n2, -1.0, Q, ldq, Q12, ldq);
+    ora_qr2_dorgqr3(m, n1, Q, ldq, T, ldt, tau);
+    return 0;
+}
+
 /* ---- checkers ------------------------------------------------------------ */
 
 /* ||triu(Q^T Q - I)||_F, the half norm of

@@ -202,6 +202,12 @@ int ora_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt,
  * (least-squares/main_lapack_ref.c:100).  tau: n doubles out. */
 int ora_dgels_hh(int m, int n, int k, double *A, int lda, double *B, int ldb, double *tau);
 
+/* the recursive QR library of qr3/src (qr2.h): qr2_dgeqr3_R.c, qr2_dlarft3.c,
+ * qr2_dorgqr3.c restated */
+int ora_qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+int ora_qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int ora_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+
 /* ---- checkers (lila-c-lila_param/lila/src/lila_test_*.c) -------------- */
 double ora_test_qq_orth_1(int m, int n, int i, const double *Q, int ldq);
 double ora_test_qq_orth_full(int m, int n, int i, const double *Q, int ldq);
