@@ -390,6 +390,9 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="run only the batched least-squares leg")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision leg and its per-kappa accuracy table")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run lila_autotune on the workload shape first (from --mt/--nx) and "
+                         "measure with the settings it picks")
     args = ap.parse_args()
     if args.gemm_tile:
         os.environ["LILA_GEMM_TILE"] = args.gemm_tile
@@ -408,6 +411,12 @@ def main():
                                    args.cpu_seconds, args.no_cpu)))
         return
     m, n, mt, nx = args.m, args.n, args.mt, args.nx
+    tuned = None
+    if args.autotune:
+        lib.lila_set_panel_mode(args.panel_mode)
+        t0 = time.perf_counter()
+        mt, nx, report = lila_amd.autotune(m, n, reps=1, mt=mt, nx=nx)
+        tuned = {"mt": mt, "nx": nx, "seconds": time.perf_counter() - t0, "report": report}
     f64 = torch.float64
     # column-major m x n storage: torch shape (n, m)
     A0 = torch.empty((n, m), dtype=f64, device=dev)
@@ -531,6 +540,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_classes": prof_all,
+        "autotune": tuned,
     }
     print(json.dumps(out))
 

@@ -269,6 +269,21 @@ void lila_dgels_batched_destroy(lila_batch_plan *plan);
 int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double *A,
                int lda, const double *B, int ldb, double beta, double *C, int ldc);
 
+/* Autotuner (no counterpart in the reference, whose mt, nx and block sizes
+ * are set by hand on its test drivers' command lines): times
+ * lila_dgeqrf_w03_recursive (VRTQ, FP64, panel dgeqr3, leaf hh) on an m x n
+ * Uniform(-1,1) device matrix and, one setting at a time from the current
+ * ones (best of `reps` runs each), picks the T block height mt from
+ * {256, 512, 1024} and the recursion cutoff nx from {32, 64, 128} (both
+ * in/out: read as the starting point, written with the choice, which the
+ * caller then passes as mt and lila_param[3]), then the GEMM tile shape,
+ * split-K occupancy target, small-output tile threshold and panel slab
+ * height, which the library keeps for the rest of the process.  `report`
+ * (may be NULL) receives the trial times and the choice.  Allocates 3 m x n
+ * matrices and a 1024 x n T on the device for the duration.  Returns 0, or
+ * -1 without a device, on bad sizes or when the allocation fails. */
+int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int len);
+
 /* Panel algorithm of the drivers (returns the previous one): 1 (default)
  * Cholesky-QR2 + Householder reconstruction in one fused launch, with the
  * column-by-column kernel as in-kernel fallback; 0 always column-by-column

@@ -81,3 +81,16 @@ def exported_names():
     txt = open(INCLUDE).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"^\s*(?:int|void|double)[\s*]+(\w+)\s*\(", txt, flags=re.M)))
+
+
+def autotune(m, n, reps=1, mt=512, nx=32):
+    """Run lila_autotune on an m x n problem from (mt, nx); returns the chosen
+    (mt, nx) and the report text.  The GEMM and panel settings it picks stay
+    in the library for the rest of the process."""
+    cmt, cnx = ctypes.c_int(mt), ctypes.c_int(nx)
+    buf = ctypes.create_string_buffer(4096)
+    r = load().lila_autotune(ctypes.c_int(m), ctypes.c_int(n), ctypes.c_int(reps),
+                             ctypes.byref(cmt), ctypes.byref(cnx), buf, ctypes.c_int(len(buf)))
+    if r != 0:
+        raise RuntimeError("lila_autotune(%d, %d) failed" % (m, n))
+    return cmt.value, cnx.value, buf.value.decode()
