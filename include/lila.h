@@ -220,6 +220,57 @@ int qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *ta
 /* qr3/src/qr2.h:27 (qr3/src/qr2_dorgqr3_VT2Q.c): the same Q */
 int qr2_dorgqr3_VT2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
 
+/* UT forms: T replaced by S = T^-1 (strict upper part of V^T V, diagonal
+ * 1/tau).  qr3/src/qr2.h:21 (qr3/src/qr2_dgeqr3_R_UT.c): V and R as
+ * qr2_dgeqr3_R, S in T */
+int qr2_dgeqr3_R_UT(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+/* ISW forms: T (or S) only on the diagonal blocks of the halving
+ * recursion's right spine (sizes n/2, (n - n/2)/2, ..., 1); the blocks
+ * between them are not written.  qr3/src/qr2.h:22-23
+ * (qr2_dgeqr3_R_ISW.c, qr2_dgeqr3_R_UT_ISW.c) */
+int qr2_dgeqr3_R_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+int qr2_dgeqr3_R_UT_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+/* qr3/src/qr2.h:26 (qr2_dorgqr3_UT.c) and :28 (qr2_dorgqr3_VS2Q.c): Q from
+ * the reflectors below Q's diagonal and S */
+int qr2_dorgqr3_UT(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+int qr2_dorgqr3_VS2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+/* qr3/src/qr2.h:31-35 (qr2_dlarft3_UT.c, _ISW.c, _UT_ISW.c, _ISW_V2T.c,
+ * _ISW_V2T_UT.c): S, or T / S on the spine blocks, from V and tau */
+int qr2_dlarft3_UT(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int qr2_dlarft3_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int qr2_dlarft3_UT_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int qr2_dlarft3_ISW_V2T(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int qr2_dlarft3_ISW_V2T_UT(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+
+/* aux routines of qr3/src (qr3/src/qr2.h:37-47) */
+/* :37 (qr2_aux_dV2N.c): upper(T) <- U U^T, U unit upper with T's strict
+ * upper part (V1^T V1 when that part holds V1^T) */
+int qr2_aux_dV2N(int n, double *T, int ldt);
+/* :38 (qr2_aux_dN2T.c): T from tau and the strict upper part of V^T V held
+ * in T (the dlarft recurrence) */
+int qr2_aux_dN2T(int n, double *tau, double *T, int ldt);
+/* :39-40 (qr2_aux_dVS2Q.c, qr2_aux_dVT2Q.c): Q (m x n) holding V below its
+ * diagonal and S (or T) on and above becomes [I;0] - V T V1^T */
+int qr2_aux_dVS2Q(int m, int n, double *Q, int ldq);
+int qr2_aux_dVT2Q(int m, int n, double *Q, int ldq);
+/* :41 (qr2_aux_dmLUinA.c): A <- -L U, L unit lower and U upper held in A */
+int qr2_aux_dmLUinA(int n, double *A, int lda);
+/* :42 (qr2_aux_dV2tau.c): tau_k = 2 / (1 + ||A(k+1:m, k)||^2); tau_{n-1} = 0
+ * when m == n */
+int qr2_aux_dV2tau(int m, int n, double *A, int lda, double *tau);
+/* :43 (qr2_aux_dULTinU.c): upper(U) <- U L^T, L unit lower (may share U's
+ * storage) */
+int qr2_aux_dULTinU(int n, double *L, int ldl, double *U, int ldu);
+/* :44 (qr2_aux_dApUBTinA.c): A (m x n) += U B^T, U m x m upper, B n x m */
+int qr2_aux_dApUBTinA(int m, int n, double *A, int lda, double *U, int ldu, double *B, int ldb);
+/* :45 (qr2_aux_dUinvLTinU.c): upper(U) <- U^-1 L^T, L unit lower */
+int qr2_aux_dUinvLTinU(int n, double *L, int ldl, double *U, int ldu);
+/* :47 (qr2_aux_dlarf_wrapper.c, LAPACK dlarf): C <- (I - tau v v^T) C for
+ * side 'L' (v of length m), C (I - tau v v^T) for 'R' (length n); v with
+ * stride incv >= 1 (-1 otherwise); work is not used */
+int qr2_aux_dlarf_wrapper(char side, int m, int n, double *V, int incv, double tau, double *C,
+                          int ldc, double *work);
+
 /* ---- extensions (no counterpart in the reference header) ------------- */
 
 /* Status of the last call (0 ok, >0 numerical info, <0 error), after

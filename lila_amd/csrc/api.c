@@ -497,6 +497,139 @@ int qr2_dorgqr3_VT2Q(int m, int n, double *Q, int ldq, double *T, int ldt, doubl
     return qr2_dorgqr3(m, n, Q, ldq, T, ldt, tau);
 }
 
+#define QR2_R_ENTRY(NAME, CALL)                                                       \
+    int NAME(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr) { \
+        BEGIN();                                                                       \
+        stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n);               \
+        const int alias = (R == A);                                                    \
+        stage_t r = alias ? a : stage_in(R, ldr, n, n);                                \
+        double *rd = alias ? a.dev : r.dev;                                            \
+        const int rl = alias ? lda : ldr;                                              \
+        (void)rd; (void)rl;                                                            \
+        CALL;                                                                          \
+        stage_out(&a, 1); stage_out(&t, 1);                                            \
+        if (!alias) stage_out(&r, 1);                                                  \
+        return api_end();                                                              \
+    }
+QR2_R_ENTRY(qr2_dgeqr3_R_UT, dv_qr2_dgeqr3_R_UT(m, n, a.dev, lda, t.dev, ldt, rd, rl))
+QR2_R_ENTRY(qr2_dgeqr3_R_ISW, dv_qr2_dgeqr3_R_isw(0, m, n, a.dev, lda, t.dev, ldt, rd, rl))
+QR2_R_ENTRY(qr2_dgeqr3_R_UT_ISW, dv_qr2_dgeqr3_R_isw(1, m, n, a.dev, lda, t.dev, ldt, rd, rl))
+#undef QR2_R_ENTRY
+
+#define QR2_T_ENTRY(NAME, CALL)                                                          \
+    int NAME(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {        \
+        BEGIN();                                                                          \
+        stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n),                   \
+                u = stage_in(tau, n, n, 1);                                               \
+        CALL;                                                                             \
+        stage_out(&a, 0); stage_out(&t, 1); stage_out(&u, 0);                             \
+        return api_end();                                                                 \
+    }
+QR2_T_ENTRY(qr2_dlarft3_UT, dv_qr2_dlarft3_UT(m, n, a.dev, lda, t.dev, ldt, u.dev))
+QR2_T_ENTRY(qr2_dlarft3_ISW, dv_qr2_dlarft3_isw(0, m, n, a.dev, lda, t.dev, ldt, u.dev))
+QR2_T_ENTRY(qr2_dlarft3_UT_ISW, dv_qr2_dlarft3_isw(1, m, n, a.dev, lda, t.dev, ldt, u.dev))
+QR2_T_ENTRY(qr2_dlarft3_ISW_V2T, dv_qr2_dlarft3_isw(0, m, n, a.dev, lda, t.dev, ldt, u.dev))
+QR2_T_ENTRY(qr2_dlarft3_ISW_V2T_UT, dv_qr2_dlarft3_isw(1, m, n, a.dev, lda, t.dev, ldt, u.dev))
+#undef QR2_T_ENTRY
+
+int qr2_dorgqr3_UT(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    BEGIN();
+    stage_t q = stage_in(Q, ldq, m, n), t = stage_in(T, ldt, n, n), u = stage_in(tau, n, n, 1);
+    dv_qr2_dorgqr3_UT(m, n, q.dev, ldq, t.dev, ldt, u.dev);
+    stage_out(&q, 1); stage_out(&t, 0); stage_out(&u, 0);
+    return api_end();
+}
+
+/* qr3/src/qr2_dorgqr3_VS2Q.c: the same Q as qr2_dorgqr3_UT */
+int qr2_dorgqr3_VS2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    return qr2_dorgqr3_UT(m, n, Q, ldq, T, ldt, tau);
+}
+
+int qr2_aux_dVT2Q(int m, int n, double *Q, int ldq) {
+    BEGIN();
+    stage_t q = stage_in(Q, ldq, m, n);
+    dv_qr2_aux_dVT2Q(m, n, q.dev, ldq);
+    stage_out(&q, 1);
+    return api_end();
+}
+
+int qr2_aux_dVS2Q(int m, int n, double *Q, int ldq) {
+    BEGIN();
+    stage_t q = stage_in(Q, ldq, m, n);
+    dv_qr2_aux_dVS2Q(m, n, q.dev, ldq);
+    stage_out(&q, 1);
+    return api_end();
+}
+
+int qr2_aux_dmLUinA(int n, double *A, int lda) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, n, n);
+    dv_qr2_aux_dmLUinA(n, a.dev, lda);
+    stage_out(&a, 1);
+    return api_end();
+}
+
+#define QR2_LU_ENTRY(NAME, DV)                                      \
+    int NAME(int n, double *L, int ldl, double *U, int ldu) {       \
+        BEGIN();                                                    \
+        const int alias = (L == U);                                 \
+        stage_t u = stage_in(U, ldu, n, n);                         \
+        stage_t l = alias ? u : stage_in(L, ldl, n, n);             \
+        DV(n, alias ? u.dev : l.dev, alias ? ldu : ldl, u.dev, ldu); \
+        stage_out(&u, 1);                                           \
+        if (!alias) stage_out(&l, 0);                               \
+        return api_end();                                           \
+    }
+QR2_LU_ENTRY(qr2_aux_dULTinU, dv_qr2_aux_dULTinU)
+QR2_LU_ENTRY(qr2_aux_dUinvLTinU, dv_qr2_aux_dUinvLTinU)
+#undef QR2_LU_ENTRY
+
+int qr2_aux_dApUBTinA(int m, int n, double *A, int lda, double *U, int ldu, double *B, int ldb) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n), u = stage_in(U, ldu, m, m), b = stage_in(B, ldb, n, m);
+    dv_qr2_aux_dApUBTinA(m, n, a.dev, lda, u.dev, ldu, b.dev, ldb);
+    stage_out(&a, 1); stage_out(&u, 0); stage_out(&b, 0);
+    return api_end();
+}
+
+int qr2_aux_dV2N(int n, double *T, int ldt) {
+    BEGIN();
+    stage_t t = stage_in(T, ldt, n, n);
+    dv_qr2_aux_dV2N(n, t.dev, ldt);
+    stage_out(&t, 1);
+    return api_end();
+}
+
+int qr2_aux_dN2T(int n, double *tau, double *T, int ldt) {
+    BEGIN();
+    stage_t t = stage_in(T, ldt, n, n), u = stage_in(tau, n, n, 1);
+    dv_qr2_aux_dN2T(n, u.dev, t.dev, ldt);
+    stage_out(&t, 1); stage_out(&u, 0);
+    return api_end();
+}
+
+int qr2_aux_dV2tau(int m, int n, double *A, int lda, double *tau) {
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n), u = stage_in(tau, n, n, 1);
+    dv_qr2_aux_dV2tau(m, n, a.dev, lda, u.dev);
+    stage_out(&a, 0); stage_out(&u, 1);
+    return api_end();
+}
+
+/* LAPACK dlarf through qr3/src/qr2_aux_dlarf_wrapper.c:3-8; work is not
+ * used; incv must be >= 1 (-1 otherwise) */
+int qr2_aux_dlarf_wrapper(char side, int m, int n, double *V, int incv, double tau, double *C,
+                          int ldc, double *work) {
+    (void)work;
+    if ((side != 'L' && side != 'R') || incv < 1) return -1;
+    BEGIN();
+    const int len = (side == 'L') ? m : n;
+    stage_t v = stage_in(V, incv, 1, len), c = stage_in(C, ldc, m, n);
+    dv_qr2_aux_dlarf(side, m, n, v.dev, incv, tau, c.dev, ldc);
+    stage_out(&v, 0); stage_out(&c, 1);
+    return api_end();
+}
+
 /* ---- least squares, generators, status --------------------------------- */
 
 int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb) {

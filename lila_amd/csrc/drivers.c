@@ -500,6 +500,227 @@ int dv_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double 
     return 0;
 }
 
+/* ---- the UT, ISW and VT2Q/VS2Q forms of qr3/src and its aux routines ----
+ *
+ * UT form: the block reflector I - V S^-1 V^T is carried by S = T^-1, whose
+ * strict upper part is that of V^T V and whose diagonal is 1/tau (the
+ * reference's UT variants build exactly that).  ISW forms: the reference's
+ * recursion computes full T (or S) blocks only for the left halves down its
+ * right spine -- diagonal blocks of sizes n/2, (n - n/2)/2, ..., 1 -- and
+ * never writes the off-diagonal blocks between them; here the full T (or S)
+ * is formed in the workspace and only those diagonal blocks are copied out. */
+
+static void c_upper_gemm(int ta, int tb, int n, int k, double alpha, const double *A, int lda, int sa,
+                         const double *B, int ldb, int sb, double *C, int ldc) {
+    ld_gemm_t g;
+    g.m = n; g.n = n; g.k = k; g.alpha = alpha; g.beta = 0.0;
+    g.A = A; g.lda = lda; g.ta = ta; g.sa = sa;
+    g.B = B; g.ldb = ldb; g.tb = tb; g.sb = sb;
+    g.C = C; g.ldc = ldc; g.zero_rows = 0; g.c_upper = 1;
+    ld_gemm(&g);
+}
+
+/* S(upper) = triu(V^T V) with diagonal 1/tau (tau[c * tinc]) */
+static void ut_from_v(int m, int n, const double *V, int ldv, const double *tau, int tinc, double *S,
+                      int lds) {
+    c_upper_gemm(T_, N_, n, m, 1.0, V, ldv, LD_VUNIT, V, ldv, LD_VUNIT, S, lds);
+    ld_set_diag(n, tau, tinc, 1, S, lds);
+}
+
+/* the diagonal blocks of the right spine of the halving recursion */
+static void copy_spine(int n, const double *W, int ldw, double *T, int ldt) {
+    int off = 0;
+    while (n > 1) {
+        const int n1 = n / 2;
+        ld_lacpy('U', n1, n1, W + off + (size_t)off * ldw, ldw, T + off + (size_t)off * ldt, ldt);
+        off += n1;
+        n -= n1;
+    }
+    if (n == 1) ld_lacpy('U', 1, 1, W + off + (size_t)off * ldw, ldw, T + off + (size_t)off * ldt, ldt);
+}
+
+/* X (n x n) = S^-1 for the upper S */
+static void inv_upper(int n, const double *S, int lds, double *X) {
+    ld_laset('A', n, n, 0.0, 1.0, X, n);
+    ld_trsm('L', 'U', 'N', 'N', n, n, 1.0, S, lds, X, n);
+}
+
+/* qr3/src/qr2_dgeqr3_R_UT.c:3-67: V and R as qr2_dgeqr3_R, T <- S */
+int dv_qr2_dgeqr3_R_UT(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr) {
+    if (n <= 0) return 0;
+    dv_qr2_dgeqr3_R(m, n, A, lda, T, ldt, R, ldr);
+    size_t mark = ld_ws_mark();
+    double *tau = wsd((size_t)n);
+    ld_lacpy('A', 1, n, T, ldt + 1, tau, 1);
+    ut_from_v(m, n, A, lda, tau, 1, T, ldt);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_dgeqr3_R_ISW.c:3-63 and qr2_dgeqr3_R_UT_ISW.c:3-64: V and R
+ * as qr2_dgeqr3_R; T (or S) on the spine blocks only */
+int dv_qr2_dgeqr3_R_isw(int ut, int m, int n, double *A, int lda, double *T, int ldt, double *R,
+                        int ldr) {
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *Tw = wsd((size_t)n * n);
+    if (ut) dv_qr2_dgeqr3_R_UT(m, n, A, lda, Tw, n, R, ldr);
+    else dv_qr2_dgeqr3_R(m, n, A, lda, Tw, n, R, ldr);
+    copy_spine(n, Tw, n, T, ldt);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_dlarft3_UT.c:3-43 */
+int dv_qr2_dlarft3_UT(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    if (n <= 0) return 0;
+    ut_from_v(m, n, A, lda, tau, 1, T, ldt);
+    return 0;
+}
+
+/* qr3/src/qr2_dlarft3_ISW.c:3-39, qr2_dlarft3_ISW_V2T.c:3-43 (ut = 0) and
+ * qr2_dlarft3_UT_ISW.c:3-39, qr2_dlarft3_ISW_V2T_UT.c:3-43 (ut = 1): the
+ * V2T forms reach the same spine blocks through V^T V */
+int dv_qr2_dlarft3_isw(int ut, int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *Tw = wsd((size_t)n * n);
+    if (ut) ut_from_v(m, n, A, lda, tau, 1, Tw, n);
+    else ld_larft(m, n, A, lda, tau, Tw, n);
+    copy_spine(n, Tw, n, T, ldt);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_dorgqr3_UT.c:3-53 and qr2_dorgqr3_VS2Q.c:3-54: Q from V (below
+ * Q's diagonal) and S = T^-1 */
+int dv_qr2_dorgqr3_UT(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    (void)tau;
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *X = wsd((size_t)n * n), *V = wsd((size_t)m * n);
+    inv_upper(n, T, ldt, X);
+    ld_lacpy('L', m, n, Q, ldq, V, m);
+    panel_q(m, n, V, m, X, n, Q, ldq);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dVT2Q.c:3-17: Q (m x n) holds V below its diagonal and T
+ * on and above; Q <- [I;0] - V T V1^T */
+int dv_qr2_aux_dVT2Q(int m, int n, double *Q, int ldq) {
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *W = wsd((size_t)m * n);
+    ld_lacpy('A', m, n, Q, ldq, W, m);
+    panel_q(m, n, W, m, W, m, Q, ldq);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dVS2Q.c:3-17: the same with S = T^-1 on and above */
+int dv_qr2_aux_dVS2Q(int m, int n, double *Q, int ldq) {
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *X = wsd((size_t)n * n), *V = wsd((size_t)m * n);
+    inv_upper(n, Q, ldq, X);
+    ld_lacpy('L', m, n, Q, ldq, V, m);
+    panel_q(m, n, V, m, X, n, Q, ldq);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dmLUinA.c:3-40: A <- -L U (L unit lower, U upper, both in A) */
+int dv_qr2_aux_dmLUinA(int n, double *A, int lda) {
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *W = wsd((size_t)n * n);
+    ld_lacpy('A', n, n, A, lda, W, n);
+    gemm(N_, N_, n, n, n, -1.0, W, n, LD_VUNIT, W, n, LD_UPPER, 0.0, A, lda);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dULTinU.c:3-38: upper(U) <- U L^T (L unit lower; U's strict
+ * lower part is not touched, so L may share U's storage) */
+int dv_qr2_aux_dULTinU(int n, double *L, int ldl, double *U, int ldu) {
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *W = wsd((size_t)n * n);
+    ld_lacpy('A', n, n, U, ldu, W, n);
+    const int alias = (L == U && ldl == ldu);
+    c_upper_gemm(N_, T_, n, n, 1.0, W, n, LD_UPPER, alias ? W : L, alias ? n : ldl, LD_VUNIT, U, ldu);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dUinvLTinU.c:3-37: upper(U) <- U^-1 L^T */
+int dv_qr2_aux_dUinvLTinU(int n, double *L, int ldl, double *U, int ldu) {
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *X = wsd((size_t)n * n);
+    ld_laset('A', n, n, 0.0, 1.0, X, n);
+    ld_trmm('R', 'L', 'T', 'U', n, n, 1.0, L, ldl, X, n);
+    ld_trsm('L', 'U', 'N', 'N', n, n, 1.0, U, ldu, X, n);
+    ld_lacpy('U', n, n, X, n, U, ldu);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dApUBTinA.c:46-96: A (m x n) += U B^T, U (m x m) upper,
+ * B n x m */
+int dv_qr2_aux_dApUBTinA(int m, int n, double *A, int lda, double *U, int ldu, double *B, int ldb) {
+    if (m <= 0 || n <= 0) return 0;
+    gemm(N_, T_, m, n, m, 1.0, U, ldu, LD_UPPER, B, ldb, LD_GEN, 1.0, A, lda);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dV2N.c:3-30: upper(T) <- U U^T for the unit upper U whose
+ * strict upper part is T's (for T holding V1^T: V1^T V1) */
+int dv_qr2_aux_dV2N(int n, double *T, int ldt) {
+    if (n <= 0) return 0;
+    size_t mark = ld_ws_mark();
+    double *W = wsd((size_t)n * n);
+    ld_lacpy('U', n, n, T, ldt, W, n);
+    c_upper_gemm(N_, T_, n, n, 1.0, W, n, LD_UUNIT, W, n, LD_UUNIT, T, ldt);
+    ld_ws_release(mark);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dN2T.c:3-40 */
+int dv_qr2_aux_dN2T(int n, double *tau, double *T, int ldt) {
+    ld_n2t(n, tau, T, ldt);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dV2tau.c:3-15 */
+int dv_qr2_aux_dV2tau(int m, int n, double *A, int lda, double *tau) {
+    if (n <= 0) return 0;
+    ld_V2tau(m, n, 0, A, lda, tau);
+    if (m == n) ld_laset('A', 1, 1, 0.0, 0.0, tau + n - 1, 1);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dlarf_wrapper.c:3-8 (LAPACK dlarf): C <- H C (side 'L')
+ * or C H ('R'), H = I - tau v v^T, v with stride incv >= 1 */
+int dv_qr2_aux_dlarf(char side, int m, int n, const double *V, int incv, double tau, double *C,
+                     int ldc) {
+    if (m <= 0 || n <= 0 || tau == 0.0) return 0;
+    const int len = (side == 'L') ? m : n;
+    size_t mark = ld_ws_mark();
+    double *v = wsd((size_t)len), *w = wsd((size_t)(side == 'L' ? n : m));
+    ld_lacpy('A', 1, len, V, incv, v, 1);
+    if (side == 'L') {
+        gemm(T_, N_, 1, n, m, 1.0, v, m, LD_GEN, C, ldc, LD_GEN, 0.0, w, 1);
+        gemm(N_, N_, m, n, 1, -tau, v, m, LD_GEN, w, 1, LD_GEN, 1.0, C, ldc);
+    } else {
+        gemm(N_, N_, m, 1, n, 1.0, C, ldc, LD_GEN, v, n, LD_GEN, 0.0, w, m);
+        gemm(N_, T_, m, n, 1, -tau, w, m, LD_GEN, v, n, LD_GEN, 1.0, C, ldc);
+    }
+    ld_ws_release(mark);
+    return 0;
+}
+
 /* least squares through (V, T, R): lila_dgels_qr3 semantic of
  * least-squares/main_ls_qr3.c:94-119.  B (m x k) overwritten, X in rows [0, n). */
 int dv_ls_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb) {

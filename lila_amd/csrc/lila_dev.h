@@ -49,6 +49,8 @@ static inline void ld_gemm_simple(int ta, int tb, int m, int n, int k, double al
 void ld_lacpy(int uplo, int m, int n, const double *A, int lda, double *B, int ldb);
 void ld_laset(int uplo, int m, int n, double offdiag, double diag, double *A, int lda);
 void ld_add_diag(int n, double *A, int lda, double v);
+/* A(i,i) = v[i*incv], or 1 / v[i*incv] when recip */
+void ld_set_diag(int n, const double *v, int incv, int recip, double *A, int lda);
 void ld_sub(int m, int n, const double *W, int ldw, double *C, int ldc); /* C -= W */
 
 /* Fused Householder panel for n <= LD_PANEL_MAX columns (ml >= n):
@@ -65,6 +67,9 @@ void ld_panel_qr_exact(int ml, int n, double *A, int lda, double *T, int ldt,
  * (ml x k, unit lower) and tau[k] (device) */
 void ld_larft(int ml, int k, const double *V, int ldv, const double *tau,
               double *T, int ldt);
+/* T (n x n upper) from the strict upper part of N = V^T V held in T and
+ * tau: the dlarft recurrence, T12 = -T11 N12 T22 between halves */
+void ld_n2t(int n, const double *tau, double *T, int ldt);
 /* copy diag of T blocks into tau (n entries, global start i) */
 void ld_T2tau(int n, int i, int mt, const double *T, int ldt, double *tau);
 /* tau_c = 2 / (1 + ||A(g+1:m, g)||^2), g = i + c */

@@ -96,5 +96,22 @@ int dq_dgeqrf_w03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, 
 int dv_qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
 int dv_qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
 int dv_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+int dv_qr2_dgeqr3_R_UT(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+int dv_qr2_dgeqr3_R_isw(int ut, int m, int n, double *A, int lda, double *T, int ldt, double *R,
+                        int ldr);
+int dv_qr2_dlarft3_UT(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int dv_qr2_dlarft3_isw(int ut, int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int dv_qr2_dorgqr3_UT(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+int dv_qr2_aux_dVT2Q(int m, int n, double *Q, int ldq);
+int dv_qr2_aux_dVS2Q(int m, int n, double *Q, int ldq);
+int dv_qr2_aux_dmLUinA(int n, double *A, int lda);
+int dv_qr2_aux_dULTinU(int n, double *L, int ldl, double *U, int ldu);
+int dv_qr2_aux_dUinvLTinU(int n, double *L, int ldl, double *U, int ldu);
+int dv_qr2_aux_dApUBTinA(int m, int n, double *A, int lda, double *U, int ldu, double *B, int ldb);
+int dv_qr2_aux_dV2N(int n, double *T, int ldt);
+int dv_qr2_aux_dN2T(int n, double *tau, double *T, int ldt);
+int dv_qr2_aux_dV2tau(int m, int n, double *A, int lda, double *tau);
+int dv_qr2_aux_dlarf(char side, int m, int n, const double *V, int incv, double tau, double *C,
+                     int ldc);
 
 #endif

@@ -355,6 +355,28 @@ void ld_larft(int ml, int k, const double *V, int ldv, const double *tau, double
   ld_ws_release(mark);
 }
 
+void ld_n2t(int n, const double *tau, double *T, int ldt) {
+  if (n <= 0) return;
+  if (n <= 32) {
+    size_t mark = ld_ws_mark();
+    double *Nm = (double *)ld_ws_alloc(sizeof(double) * (size_t)n * n);
+    ld_lacpy('U', n, n, T, ldt, Nm, n);
+    hipLaunchKernelGGL(larft_small_kernel, dim3(1), dim3(32), 0, S(), n, Nm, n, tau, T, ldt);
+    ld_ws_release(mark);
+    return;
+  }
+  const int n1 = n / 2, n2 = n - n1;
+  double *T22 = T + n1 + (size_t)n1 * ldt, *T12 = T + (size_t)n1 * ldt;
+  ld_n2t(n1, tau, T, ldt);
+  ld_n2t(n2, tau + n1, T22, ldt);
+  // T12 = -T11 N12 T22
+  size_t mark = ld_ws_mark();
+  double *H = (double *)ld_ws_alloc(sizeof(double) * (size_t)n1 * n2);
+  gemm(0, 0, n1, n2, n1, 1.0, T, ldt, LD_UPPER, T12, ldt, LD_GEN, 0.0, H, n1);
+  gemm(0, 0, n1, n2, n2, -1.0, H, n1, LD_GEN, T22, ldt, LD_UPPER, 0.0, T12, ldt);
+  ld_ws_release(mark);
+}
+
 // Sign-chosen unpivoted LU of the mr x nc panel A (mr >= nc), recursive over
 // the columns: the left half, U12 = L11^-1 A12, A22 -= L21 U12 (FP64 MFMA),
 // the right half; column k's negation (S_k = -1) reaches its U12 entries

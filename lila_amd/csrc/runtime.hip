@@ -400,6 +400,15 @@ __global__ void add_diag_kernel(int n, double *A, int lda, double v) {
   if (i < n) A[(size_t)i + (size_t)i * lda] += v;
 }
 
+__global__ void set_diag_kernel(int n, const double *__restrict__ v, int incv, int recip, double *A,
+                                int lda) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double x = v[(size_t)i * incv];
+    A[(size_t)i + (size_t)i * lda] = recip ? 1.0 / x : x;
+  }
+}
+
 __global__ void sub_kernel(int m, int n, const double *__restrict__ W, int ldw, double *C, int ldc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
@@ -507,6 +516,11 @@ void ld_laset(int uplo, int m, int n, double off, double dg, double *A, int lda)
 void ld_add_diag(int n, double *A, int lda, double v) {
   if (n <= 0) return;
   hipLaunchKernelGGL(add_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, g.stream, n, A, lda, v);
+}
+void ld_set_diag(int n, const double *v, int incv, int recip, double *A, int lda) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(set_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, g.stream, n, v, incv, recip,
+                     A, lda);
 }
 void ld_sub(int m, int n, const double *W, int ldw, double *C, int ldc) {
   if (m <= 0 || n <= 0) return;

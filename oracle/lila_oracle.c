@@ -1648,6 +1648,324 @@ int ora_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double
     return 0;
 }
 
+/* The dgeqr3_R family with the form flags: ut = 1 carries S = T^-1
+ * (qr3/src/qr2_dgeqr3_R_UT.c:3-67: leaf S = 1/tau, the update solves with
+ * S11^T where the T form multiplies by T11^T, and the off-diagonal block
+ * keeps V1^T V2); isw = 1 skips the off-diagonal block, the left half
+ * running the non-ISW routine of the same form (qr2_dgeqr3_R_ISW.c:3-63,
+ * qr2_dgeqr3_R_UT_ISW.c:3-64).  form(0, 0) is ora_qr2_dgeqr3_R. */
+static void o_geqr3_R_form(int ut, int isw, int m, int n, double *A, int lda, double *T, int ldt,
+                           double *R, int ldr) {
+    if (n == 1) {
+        R[0] = A[0];
+        ora_dlarfg(m, R, A + 1, 1, T);
+        if (ut) T[0] = 1.0 / T[0];
+        return;
+    }
+    const int n1 = n / 2, n2 = n - n1;
+    double *A12 = A + (size_t)n1 * lda, *A21 = A + n1, *A22 = A + n1 + (size_t)n1 * lda;
+    double *T12 = T + (size_t)n1 * ldt, *T22 = T + n1 + (size_t)n1 * ldt;
+    double *R12 = R + (size_t)n1 * ldr, *R22 = R + n1 + (size_t)n1 * ldr;
+    o_geqr3_R_form(ut, 0, m, n1, A, lda, T, ldt, R, ldr);
+    double *W = (A == R) ? T12 : R12;
+    const int ldw = (A == R) ? ldt : ldr;
+    ora_dlacpy('A', n1, n2, A12, lda, W, ldw);
+    ora_dtrmm('L', 'L', 'T', 'U', n1, n2, 1.0, A, lda, W, ldw);
+    ora_dgemm('T', 'N', n1, n2, m - n1, 1.0, A21, lda, A22, lda, 1.0, W, ldw);
+    if (ut) ora_dtrsm('L', 'U', 'T', 'N', n1, n2, 1.0, T, ldt, W, ldw);
+    else ora_dtrmm('L', 'U', 'T', 'N', n1, n2, 1.0, T, ldt, W, ldw);
+    ora_dgemm('N', 'N', m - n1, n2, n1, -1.0, A21, lda, W, ldw, 1.0, A22, lda);
+    ora_dtrmm('L', 'L', 'N', 'U', n1, n2, 1.0, A, lda, W, ldw);
+    for (int c = 0; c < n2; c++)
+        for (int r = 0; r < n1; r++) IDX(R12, r, c, ldr) = IDX(A12, r, c, lda) - IDX(W, r, c, ldw);
+    o_geqr3_R_form(ut, isw, m - n1, n2, A22, lda, T22, ldt, R22, ldr);
+    if (isw) return;
+    for (int r = 0; r < n1; r++)
+        for (int c = 0; c < n2; c++) IDX(T12, r, c, ldt) = IDX(A21, c, r, lda);
+    ora_dtrmm('R', 'L', 'N', 'U', n1, n2, 1.0, A22, lda, T12, ldt);
+    ora_dgemm('T', 'N', n1, n2, m - n, 1.0, A21 + n2, lda, A22 + n2, lda, 1.0, T12, ldt);
+    if (!ut) {
+        ora_dtrmm('L', 'U', 'N', 'N', n1, n2, -1.0, T, ldt, T12, ldt);
+        ora_dtrmm('R', 'U', 'N', 'N', n1, n2, 1.0, T22, ldt, T12, ldt);
+    }
+}
+
+int ora_qr2_dgeqr3_R_UT(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr) {
+    o_geqr3_R_form(1, 0, m, n, A, lda, T, ldt, R, ldr);
+    return 0;
+}
+int ora_qr2_dgeqr3_R_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr) {
+    o_geqr3_R_form(0, 1, m, n, A, lda, T, ldt, R, ldr);
+    return 0;
+}
+int ora_qr2_dgeqr3_R_UT_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *R,
+                            int ldr) {
+    o_geqr3_R_form(1, 1, m, n, A, lda, T, ldt, R, ldr);
+    return 0;
+}
+
+/* The dlarft3 family: ut / isw as above (qr2_dlarft3_UT.c:3-43,
+ * qr2_dlarft3_ISW.c:3-39, qr2_dlarft3_UT_ISW.c:3-39); v2t = 1 builds each
+ * left half from V^T V instead of recursing (qr2_dlarft3_ISW_V2T.c:3-43,
+ * qr2_dlarft3_ISW_V2T_UT.c:3-43: V11^T into the strict upper part, dV2N,
+ * the Gram of the rows below, then dN2T or the 1/tau diagonal). */
+static void o_larft3_form(int ut, int isw, int v2t, int m, int n, double *A, int lda, double *T,
+                          int ldt, double *tau) {
+    if (n <= 1) {
+        T[0] = ut ? 1.0 / tau[0] : tau[0];
+        return;
+    }
+    const int n1 = n / 2, n2 = n - n1;
+    double *A21 = A + n1, *A31 = A + n, *A22 = A + n1 + (size_t)n1 * lda, *A32 = A + n + (size_t)n1 * lda;
+    double *T12 = T + (size_t)n1 * ldt, *T22 = T + n1 + (size_t)n1 * ldt;
+    if (v2t) {
+        for (int c = 0; c < n1; c++)
+            for (int r = 0; r < c; r++) IDX(T, r, c, ldt) = IDX(A, c, r, lda);
+        ora_qr2_aux_dV2N(n1, T, ldt);
+        ora_dsyrk('U', 'T', n1, m - n1, 1.0, A + n1, lda, 1.0, T, ldt);
+        if (ut)
+            for (int c = 0; c < n1; c++) IDX(T, c, c, ldt) = 1.0 / tau[c];
+        else
+            ora_qr2_aux_dN2T(n1, tau, T, ldt);
+    } else {
+        o_larft3_form(ut, 0, 0, m, n1, A, lda, T, ldt, tau);
+    }
+    o_larft3_form(ut, isw, v2t, m - n1, n2, A22, lda, T22, ldt, tau + n1);
+    if (isw) return;
+    for (int r = 0; r < n1; r++)
+        for (int c = 0; c < n2; c++) IDX(T12, r, c, ldt) = IDX(A21, c, r, lda);
+    ora_dtrmm('R', 'L', 'N', 'U', n1, n2, 1.0, A22, lda, T12, ldt);
+    ora_dgemm('T', 'N', n1, n2, m - n, 1.0, A31, lda, A32, lda, 1.0, T12, ldt);
+    if (!ut) {
+        ora_dtrmm('L', 'U', 'N', 'N', n1, n2, -1.0, T, ldt, T12, ldt);
+        ora_dtrmm('R', 'U', 'N', 'N', n1, n2, 1.0, T22, ldt, T12, ldt);
+    }
+}
+
+int ora_qr2_dlarft3_UT(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    o_larft3_form(1, 0, 0, m, n, A, lda, T, ldt, tau);
+    return 0;
+}
+int ora_qr2_dlarft3_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    o_larft3_form(0, 1, 0, m, n, A, lda, T, ldt, tau);
+    return 0;
+}
+int ora_qr2_dlarft3_UT_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    o_larft3_form(1, 1, 0, m, n, A, lda, T, ldt, tau);
+    return 0;
+}
+int ora_qr2_dlarft3_ISW_V2T(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    o_larft3_form(0, 1, 1, m, n, A, lda, T, ldt, tau);
+    return 0;
+}
+int ora_qr2_dlarft3_ISW_V2T_UT(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
+    o_larft3_form(1, 1, 1, m, n, A, lda, T, ldt, tau);
+    return 0;
+}
+
+/* The dorgqr3 family: ut solves with S11 where the T form multiplies by T11
+ * (qr2_dorgqr3_UT.c:3-53); aux = 1 hands each left half to dVT2Q / dVS2Q
+ * after copying its T (S) block into Q's upper part (qr2_dorgqr3_VT2Q.c:3-54,
+ * qr2_dorgqr3_VS2Q.c:3-54).  form(0, 0) is ora_qr2_dorgqr3. */
+static void o_orgqr3_form(int ut, int aux, int m, int n, double *Q, int ldq, double *T, int ldt,
+                          double *tau) {
+    if (n <= 1) {
+        for (int r = 1; r < m; r++) Q[r] *= -tau[0];
+        Q[0] = 1.0 - tau[0];
+        return;
+    }
+    const int n1 = n / 2, n2 = n - n1;
+    double *Q21 = Q + n1, *Q12 = Q + (size_t)n1 * ldq, *Q22 = Q + n1 + (size_t)n1 * ldq;
+    o_orgqr3_form(ut, aux, m - n1, n2, Q22, ldq, T + n1 + (size_t)n1 * ldt, ldt, tau + n1);
+    ora_dgemm('T', 'N', n1, n2, m - n1, 1.0, Q21, ldq, Q22, ldq, 0.0, Q12, ldq);
+    if (ut) ora_dtrsm('L', 'U', 'N', 'N', n1, n2, 1.0, T, ldt, Q12, ldq);
+    else ora_dtrmm('L', 'U', 'N', 'N', n1, n2, 1.0, T, ldt, Q12, ldq);
+    ora_dgemm('N', 'N', m - n1, n2, n1, -1.0, Q21, ldq, Q12, ldq, 1.0, Q22, ldq);
+    ora_dtrmm('L', 'L', 'N', 'U', n1, n2, -1.0, Q, ldq, Q12, ldq);
+    if (aux) {
+        if (T != Q) ora_dlacpy('U', n1, n1, T, ldt, Q, ldq);
+        if (ut) ora_qr2_aux_dVS2Q(m, n1, Q, ldq);
+        else ora_qr2_aux_dVT2Q(m, n1, Q, ldq);
+    } else {
+        o_orgqr3_form(ut, 0, m, n1, Q, ldq, T, ldt, tau);
+    }
+}
+
+int ora_qr2_dorgqr3_UT(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    o_orgqr3_form(1, 0, m, n, Q, ldq, T, ldt, tau);
+    return 0;
+}
+int ora_qr2_dorgqr3_VT2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    o_orgqr3_form(0, 1, m, n, Q, ldq, T, ldt, tau);
+    return 0;
+}
+int ora_qr2_dorgqr3_VS2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
+    o_orgqr3_form(1, 1, m, n, Q, ldq, T, ldt, tau);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dV2N.c:3-30: upper(T) <- U U^T, U = I + strict upper of
+ * T, by halves: T11 <- U11 U11^T + T12 T12^T, T12 <- T12 U22^T */
+int ora_qr2_aux_dV2N(int n, double *T, int ldt) {
+    if (n <= 1) {
+        T[0] = 1.0;
+        return 0;
+    }
+    const int n1 = n / 2, n2 = n - n1;
+    double *T12 = T + (size_t)n1 * ldt, *T22 = T + n1 + (size_t)n1 * ldt;
+    ora_qr2_aux_dV2N(n1, T, ldt);
+    ora_dsyrk('U', 'N', n1, n2, 1.0, T12, ldt, 1.0, T, ldt);
+    ora_dtrmm('R', 'U', 'T', 'U', n1, n2, 1.0, T22, ldt, T12, ldt);
+    ora_qr2_aux_dV2N(n2, T22, ldt);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dN2T.c:3-40: T from tau and N (strict upper part of
+ * V^T V) by halves, T12 <- -T11 N12 T22 */
+int ora_qr2_aux_dN2T(int n, double *tau, double *T, int ldt) {
+    if (n <= 1) {
+        T[0] = tau[0];
+        return 0;
+    }
+    const int n1 = n / 2, n2 = n - n1;
+    double *T12 = T + (size_t)n1 * ldt, *T22 = T + n1 + (size_t)n1 * ldt;
+    ora_qr2_aux_dN2T(n1, tau, T, ldt);
+    ora_qr2_aux_dN2T(n2, tau + n1, T22, ldt);
+    ora_dtrmm('L', 'U', 'N', 'N', n1, n2, -1.0, T, ldt, T12, ldt);
+    ora_dtrmm('R', 'U', 'N', 'N', n1, n2, 1.0, T22, ldt, T12, ldt);
+    return 0;
+}
+
+/* tail shared by qr3/src/qr2_aux_dVT2Q.c:3-17 and qr2_aux_dVS2Q.c:3-17 once
+ * the top block's upper part holds X = T V1^T: Q2 <- -V2 X, Q1 <- I - V1 X */
+static void o_vx2q(int m, int n, double *Q, int ldq) {
+    ora_dtrmm('R', 'U', 'N', 'N', m - n, n, -1.0, Q, ldq, Q + n, ldq);
+    ora_qr2_aux_dmLUinA(n, Q, ldq);
+    for (int k = 0; k < n; k++) IDX(Q, k, k, ldq) += 1.0;
+}
+
+int ora_qr2_aux_dVT2Q(int m, int n, double *Q, int ldq) {
+    ora_qr2_aux_dULTinU(n, Q, ldq, Q, ldq);
+    o_vx2q(m, n, Q, ldq);
+    return 0;
+}
+
+int ora_qr2_aux_dVS2Q(int m, int n, double *Q, int ldq) {
+    ora_qr2_aux_dUinvLTinU(n, Q, ldq, Q, ldq);
+    o_vx2q(m, n, Q, ldq);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dmLUinA.c:3-40: A <- -L U (L unit lower, U upper, both in
+ * A), bottom-right first: A22 <- -L22 U22 - L21 U12, A12 <- -L11 U12,
+ * A21 <- -L21 U11, then A11 */
+int ora_qr2_aux_dmLUinA(int n, double *A, int lda) {
+    if (n <= 1) {
+        A[0] = -A[0];
+        return 0;
+    }
+    const int n1 = n / 2, n2 = n - n1;
+    double *A12 = A + (size_t)n1 * lda, *A21 = A + n1, *A22 = A + n1 + (size_t)n1 * lda;
+    ora_qr2_aux_dmLUinA(n2, A22, lda);
+    ora_dgemm('N', 'N', n2, n2, n1, -1.0, A21, lda, A12, lda, 1.0, A22, lda);
+    ora_dtrmm('L', 'L', 'N', 'U', n1, n2, -1.0, A, lda, A12, lda);
+    ora_dtrmm('R', 'U', 'N', 'N', n2, n1, -1.0, A, lda, A21, lda);
+    ora_qr2_aux_dmLUinA(n1, A, lda);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dV2tau.c:3-15 */
+int ora_qr2_aux_dV2tau(int m, int n, double *A, int lda, double *tau) {
+    for (int k = 0; k < n; k++) {
+        double s = 1.0;
+        for (int r = k + 1; r < m; r++) s += IDX(A, r, k, lda) * IDX(A, r, k, lda);
+        tau[k] = 2.0 / s;
+    }
+    if (m == n && n > 0) tau[m - 1] = 0.0;
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dULTinU.c:3-38: upper(U) <- U L^T by halves:
+ * U12 <- U12 L22^T + U11 L21^T, then U11 and U22 */
+int ora_qr2_aux_dULTinU(int n, double *L, int ldl, double *U, int ldu) {
+    if (n <= 1) return 0;
+    const int n1 = n / 2, n2 = n - n1;
+    double *U12 = U + (size_t)n1 * ldu, *U22 = U + n1 + (size_t)n1 * ldu;
+    double *L21 = L + n1, *L22 = L + n1 + (size_t)n1 * ldl;
+    ora_dtrmm('R', 'L', 'T', 'U', n1, n2, 1.0, L22, ldl, U12, ldu);
+    ora_qr2_aux_dApUBTinA(n1, n2, U12, ldu, U, ldu, L21, ldl);
+    ora_qr2_aux_dULTinU(n1, L, ldl, U, ldu);
+    ora_qr2_aux_dULTinU(n2, L22, ldl, U22, ldu);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dApUBTinA.c:46-96: A (m x n) += U B^T with U upper (m x m)
+ * and B n x m, by halves of both dimensions */
+int ora_qr2_aux_dApUBTinA(int m, int n, double *A, int lda, double *U, int ldu, double *B, int ldb) {
+    if (m <= 0 || n <= 0) return 0;
+    if (m <= 1 && n <= 1) {
+        A[0] += U[0] * B[0];
+        return 0;
+    }
+    const int m1 = m / 2, m2 = m - m1, n1 = n / 2, n2 = n - n1;
+    double *A12 = A + (size_t)n1 * lda, *A21 = A + m1, *A22 = A + m1 + (size_t)n1 * lda;
+    double *B12 = B + (size_t)m1 * ldb, *B21 = B + n1, *B22 = B + n1 + (size_t)m1 * ldb;
+    double *U12 = U + (size_t)m1 * ldu, *U22 = U + m1 + (size_t)m1 * ldu;
+    if (m1 > 0 && n1 > 0) ora_qr2_aux_dApUBTinA(m1, n1, A, lda, U, ldu, B, ldb);
+    if (m1 > 0 && n1 > 0 && m2 > 0)
+        ora_dgemm('N', 'T', m1, n1, m2, 1.0, U12, ldu, B12, ldb, 1.0, A, lda);
+    if (m1 > 0 && n2 > 0) ora_qr2_aux_dApUBTinA(m1, n2, A12, lda, U, ldu, B21, ldb);
+    if (m1 > 0 && n2 > 0 && m2 > 0)
+        ora_dgemm('N', 'T', m1, n2, m2, 1.0, U12, ldu, B22, ldb, 1.0, A12, lda);
+    if (m2 > 0 && n1 > 0) ora_qr2_aux_dApUBTinA(m2, n1, A21, lda, U22, ldu, B12, ldb);
+    if (m2 > 0 && n2 > 0) ora_qr2_aux_dApUBTinA(m2, n2, A22, lda, U22, ldu, B22, ldb);
+    return 0;
+}
+
+/* qr3/src/qr2_aux_dUinvLTinU.c:3-37: upper(U) <- U^-1 L^T, bottom-right
+ * first: U12 <- U11^-1 (L21^T - U12 X22) with X22 the new U22 */
+int ora_qr2_aux_dUinvLTinU(int n, double *L, int ldl, double *U, int ldu) {
+    if (n <= 1) {
+        U[0] = 1.0 / U[0];
+        return 0;
+    }
+    const int n1 = n / 2, n2 = n - n1;
+    double *U12 = U + (size_t)n1 * ldu, *U22 = U + n1 + (size_t)n1 * ldu;
+    double *L21 = L + n1, *L22 = L + n1 + (size_t)n1 * ldl;
+    ora_qr2_aux_dUinvLTinU(n2, L22, ldl, U22, ldu);
+    ora_dtrmm('R', 'U', 'N', 'N', n1, n2, -1.0, U22, ldu, U12, ldu);
+    for (int r = 0; r < n1; r++)
+        for (int c = 0; c < n2; c++) IDX(U12, r, c, ldu) += IDX(L21, c, r, ldl);
+    ora_dtrsm('L', 'U', 'N', 'N', n1, n2, 1.0, U, ldu, U12, ldu);
+    ora_qr2_aux_dUinvLTinU(n1, L, ldl, U, ldu);
+    return 0;
+}
+
+/* LAPACK dlarf as called by qr3/src/qr2_aux_dlarf_wrapper.c:3-8, v with
+ * stride incv >= 1: C <- (I - tau v v^T) C ('L') or C (I - tau v v^T) ('R') */
+int ora_qr2_aux_dlarf_wrapper(char side, int m, int n, double *V, int incv, double tau, double *C,
+                              int ldc, double *work) {
+    (void)work;
+    if ((side != 'L' && side != 'R') || incv < 1) return -1;
+    if (tau == 0.0 || m <= 0 || n <= 0) return 0;
+    if (side == 'L') {
+        for (int c = 0; c < n; c++) {
+            double s = 0.0;
+            for (int r = 0; r < m; r++) s += V[(size_t)r * incv] * IDX(C, r, c, ldc);
+            s *= tau;
+            for (int r = 0; r < m; r++) IDX(C, r, c, ldc) -= s * V[(size_t)r * incv];
+        }
+    } else {
+        for (int r = 0; r < m; r++) {
+            double s = 0.0;
+            for (int c = 0; c < n; c++) s += IDX(C, r, c, ldc) * V[(size_t)c * incv];
+            s *= tau;
+            for (int c = 0; c < n; c++) IDX(C, r, c, ldc) -= s * V[(size_t)c * incv];
+        }
+    }
+    return 0;
+}
+
 /* ---- checkers ------------------------------------------------------------ */
 
 /* ||triu(Q^T Q - I)||_F, the half norm of

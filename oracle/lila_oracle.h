@@ -207,6 +207,30 @@ int ora_dgels_hh(int m, int n, int k, double *A, int lda, double *B, int ldb, do
 int ora_qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
 int ora_qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
 int ora_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+/* the UT / ISW / V2T / VT2Q / VS2Q forms and the aux routines of qr3/src */
+int ora_qr2_dgeqr3_R_UT(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+int ora_qr2_dgeqr3_R_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
+int ora_qr2_dgeqr3_R_UT_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *R,
+                            int ldr);
+int ora_qr2_dlarft3_UT(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int ora_qr2_dlarft3_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int ora_qr2_dlarft3_UT_ISW(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int ora_qr2_dlarft3_ISW_V2T(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int ora_qr2_dlarft3_ISW_V2T_UT(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
+int ora_qr2_dorgqr3_UT(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+int ora_qr2_dorgqr3_VT2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+int ora_qr2_dorgqr3_VS2Q(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+int ora_qr2_aux_dV2N(int n, double *T, int ldt);
+int ora_qr2_aux_dN2T(int n, double *tau, double *T, int ldt);
+int ora_qr2_aux_dVS2Q(int m, int n, double *Q, int ldq);
+int ora_qr2_aux_dVT2Q(int m, int n, double *Q, int ldq);
+int ora_qr2_aux_dmLUinA(int n, double *A, int lda);
+int ora_qr2_aux_dV2tau(int m, int n, double *A, int lda, double *tau);
+int ora_qr2_aux_dULTinU(int n, double *L, int ldl, double *U, int ldu);
+int ora_qr2_aux_dApUBTinA(int m, int n, double *A, int lda, double *U, int ldu, double *B, int ldb);
+int ora_qr2_aux_dUinvLTinU(int n, double *L, int ldl, double *U, int ldu);
+int ora_qr2_aux_dlarf_wrapper(char side, int m, int n, double *V, int incv, double tau, double *C,
+                              int ldc, double *work);
 
 /* ---- checkers (lila-c-lila_param/lila/src/lila_test_*.c) -------------- */
 double ora_test_qq_orth_1(int m, int n, int i, const double *Q, int ldq);
