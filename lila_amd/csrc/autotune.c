@@ -129,6 +129,24 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
         else { tu->gemm_bm = bm0; tu->gemm_bn = bn0; }
     }
     {
+        /* the tile of the products with a transposed A (W = V^T C) */
+        static const int tiles[][2] = {{128, 128}, {64, 128}, {128, 64}};
+        int keep = -1;
+        for (int k = 0; k < 3; k++)
+            if (tiles[k][0] == tu->gemm_bm_t && tiles[k][1] == tu->gemm_bn_t) keep = k;
+        const int bm0 = tu->gemm_bm_t, bn0 = tu->gemm_bn_t;
+        for (int k = 0; k < 3; k++) {
+            if (k == keep) continue;
+            tu->gemm_bm_t = tiles[k][0]; tu->gemm_bn_t = tiles[k][1];
+            const double t = time_qr(&c, c.reps);
+            c.trials++;
+            logf_(&c, "; tile_t %dx%d %.3f", tiles[k][0], tiles[k][1], t * 1e3);
+            if (t < c.best) { c.best = t; keep = k; }
+        }
+        if (keep >= 0) { tu->gemm_bm_t = tiles[keep][0]; tu->gemm_bn_t = tiles[keep][1]; }
+        else { tu->gemm_bm_t = bm0; tu->gemm_bn_t = bn0; }
+    }
+    {
         static const int v[] = {256, 512, 1024};
         sweep(&c, "min_wgs", &tu->gemm_min_wgs, v, 3);
     }
@@ -140,10 +158,10 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
         static const int v[] = {96, 128, 192, 256};
         sweep(&c, "panel_rows", &tu->panel_rows_cqr, v, 4);
     }
-    logf_(&c, "; chose mt %d nx %d tile %dx%d min_wgs %d small_tiles %d panel_rows %d: "
-              "%.3f ms (%d trials)",
-          c.mt, c.nx, tu->gemm_bm, tu->gemm_bn, tu->gemm_min_wgs, tu->gemm_small_tiles,
-          tu->panel_rows_cqr, c.best * 1e3, c.trials);
+    logf_(&c, "; chose mt %d nx %d tile %dx%d tile_t %dx%d min_wgs %d small_tiles %d "
+              "panel_rows %d: %.3f ms (%d trials)",
+          c.mt, c.nx, tu->gemm_bm, tu->gemm_bn, tu->gemm_bm_t, tu->gemm_bn_t, tu->gemm_min_wgs,
+          tu->gemm_small_tiles, tu->panel_rows_cqr, c.best * 1e3, c.trials);
     *mt = c.mt;
     *nx = c.nx;
     ld_dev_free(A0); ld_dev_free(A); ld_dev_free(Q); ld_dev_free(T);

@@ -218,7 +218,24 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
   // `lane`, register v (rows interleaved by 4), D column lane%16; the f32
   // 16x16x4 one puts row 4 (lane/16) + v there (four consecutive rows).
   // D = C^T here, so lane holds C(row = base + lane%16, col = base + D row)
+  // Every C load of the tile is issued before the first store, so the tile
+  // pays one memory round trip instead of one per element.
   const int rl = lane & 15, cg = lane >> 4;
+  const bool read_c = (P == nullptr) && beta != 0.0;
+  double cv[WMT][WNT][4];
+#pragma unroll
+  for (int a = 0; a < WMT; a++) {
+    const int gi = m0 + wm * (BM / 2) + a * 16 + rl;
+#pragma unroll
+    for (int b = 0; b < WNT; b++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = n0 + wn * (BN / 2) + b * 16 + (F32 ? 4 * cg + v : cg + 4 * v);
+        const bool ld = read_c && gi < M && gj < N && gi >= zero_rows && !(c_upper && gi > gj);
+        cv[a][b][v] = ld ? C[(size_t)gi + (size_t)gj * ldc] : 0.0;
+      }
+    }
+  }
 #pragma unroll
   for (int a = 0; a < WMT; a++) {
     const int gi = m0 + wm * (BM / 2) + a * 16 + rl;
@@ -233,10 +250,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
         if (P) {
           P[(size_t)blockIdx.z * M * N + (size_t)gi + (size_t)gj * M] = r;
         } else {
-          double *c = &C[(size_t)gi + (size_t)gj * ldc];
           double o = alpha * r;
-          if (beta != 0.0 && gi >= zero_rows) o += beta * (*c);
-          *c = o;
+          if (read_c && gi >= zero_rows) o += beta * cv[a][b][v];
+          C[(size_t)gi + (size_t)gj * ldc] = o;
         }
       }
     }
@@ -325,10 +341,16 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
   }
   const int M = g->m, N = g->n, K = g->k;
   const ld_tune_t *tu = ld_tune();
-  int bm = tu->gemm_bm, bn = tu->gemm_bn;
+  int bm = g->ta ? tu->gemm_bm_t : tu->gemm_bm, bn = g->ta ? tu->gemm_bn_t : tu->gemm_bn;
   if (M <= 32) bm = 32; else if (M <= 64 && bm > 64) bm = 64;
   if (N <= 32) bn = 32; else if (N <= 64 && bn > 64) bn = 64;
   long tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
+  // too few tiles to fill the chip: step down to 64 x 128 (from a larger
+  // tile), then to 64 x 64
+  if (tiles < tu->gemm_small_tiles && bm * bn > 64 * 128 && N > 64) {
+    bm = 64; bn = 128;
+    tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
+  }
   if (tiles < tu->gemm_small_tiles && bm * bn > 64 * 64) {
     bm = 64; bn = 64;
     tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);

@@ -148,6 +148,7 @@ typedef struct {
     int gemm_small_tiles;    /* below this many large tiles use 64 x 64 */
     int panel_mode;          /* 0 column Householder, 1 Cholesky-QR2 + reconstruction */
     int gemm_lowp;           /* 1: GEMMs on FP32 MFMA (operands rounded to FP32) */
+    int gemm_bm_t, gemm_bn_t; /* tile of products with op(A) = A^T (W = V^T C) */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

@@ -38,7 +38,7 @@ struct State {
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
-  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0};
+  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128};
 };
 State g;
 }  // namespace
@@ -70,6 +70,15 @@ int ld_init(void) {
         (bn == 32 || bn == 64 || bn == 128)) {
       g.tune.gemm_bm = bm;
       g.tune.gemm_bn = bn;
+    }
+  }
+  e = getenv("LILA_GEMM_TILE_T");  // the same for products with op(A) = A^T
+  if (e) {
+    int bm = 0, bn = 0;
+    if (sscanf(e, "%dx%d", &bm, &bn) == 2 && (bm == 32 || bm == 64 || bm == 128) &&
+        (bn == 32 || bn == 64 || bn == 128)) {
+      g.tune.gemm_bm_t = bm;
+      g.tune.gemm_bn_t = bn;
     }
   }
   e = getenv("LILA_GEMM_SMALL_TILES");
