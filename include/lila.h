@@ -341,6 +341,16 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
  * (dgeqr3's arithmetic).  Both give dgeqr3's V, R, T and Q to rounding. */
 int lila_set_panel_mode(int mode);
 
+/* Streams of lila_dgeqrf_w03_recursive (returns the previous count): 1
+ * (default) keeps every step on the library stream; 2 builds Q (the
+ * dormqrbz steps) on a second, internal stream, overlapped with the
+ * factorization on the library stream and joined back before the call's
+ * work on the library stream ends (about 2% faster at m = n = 16384; the
+ * overlapped launches then share the GPU, so per-launch timings stretch).
+ * The environment variable LILA_LANES=2 sets 2 at initialisation.  Results
+ * agree to rounding either way. */
+int lila_set_lanes(int n);
+
 /* Arithmetic of the library's GEMMs (returns the previous setting): 0
  * (default) FP64 MFMA; 1 FP32 MFMA with FP32 accumulation, operands rounded
  * to FP32 (the low-precision updates of the mixed-precision QR, selected by

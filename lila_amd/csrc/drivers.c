@@ -413,16 +413,42 @@ int dv_dgeqrf_mixed(const int *p, int m, int n, int i, int mt, double *A, int ld
 
 /* lila-c/lila_dgeqrf_w03_recursive.c:3-29; lila_param[0] == 1 selects the
  * mixed-precision form above */
+/* The Q-building step of a node (dormqrbz) reads the node's left reflectors
+ * (V below the diagonal of columns [i, i+n1), their T blocks) and the right
+ * half's Q columns, and writes Q's columns [i+n1, i+n).  Nothing the
+ * factorization does afterwards touches those: it writes A and T only in
+ * columns further right (the parent's connect writes T rows above the
+ * diagonal blocks read here) and Q only through its own later leaves and
+ * Q-building steps.  So the Q-building steps run on lane 1, each after a
+ * fork that orders it behind the node's factorization on lane 0, and the
+ * outermost call joins lane 1 back before it returns. */
+static int w03_depth = 0;
+
+static int w03_rec(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T, int ldt,
+                   double *Q, int ldq) {
+    if (n <= p[3]) return dv_dgeqrf_w03_mt(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    int n1 = n / 2, n2 = n - n1;
+    w03_rec(p, m, n1, i, mt, A, lda, T, ldt, Q, ldq);
+    dv_dormqrf_w03(m, n2, n1, i, i + n1, mt, A, lda, T, ldt);
+    w03_rec(p, m, n2, i + n1, mt, A, lda, T, ldt, Q, ldq);
+    dv_dlarft_connect_w03(m, n2, i + n1, i, mt, A, lda, T, ldt);
+    if (ld_lanes() > 1) {
+        ld_lane_fork();
+        ld_set_lane(1);
+        dv_dormqrbz_w03(m, n2, n1, i, i + n1, mt, A, lda, Q, ldq, T, ldt);
+        ld_set_lane(0);
+    } else {
+        dv_dormqrbz_w03(m, n2, n1, i, i + n1, mt, A, lda, Q, ldq, T, ldt);
+    }
+    return 0;
+}
+
 int dv_dgeqrf_w03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
                             int ldt, double *Q, int ldq) {
     if (p[0] == 1) return dv_dgeqrf_mixed(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
-    if (n <= p[3]) return dv_dgeqrf_w03_mt(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
-    int n1 = n / 2, n2 = n - n1;
-    dv_dgeqrf_w03_recursive(p, m, n1, i, mt, A, lda, T, ldt, Q, ldq);
-    dv_dormqrf_w03(m, n2, n1, i, i + n1, mt, A, lda, T, ldt);
-    dv_dgeqrf_w03_recursive(p, m, n2, i + n1, mt, A, lda, T, ldt, Q, ldq);
-    dv_dlarft_connect_w03(m, n2, i + n1, i, mt, A, lda, T, ldt);
-    dv_dormqrbz_w03(m, n2, n1, i, i + n1, mt, A, lda, Q, ldq, T, ldt);
+    w03_depth++;
+    w03_rec(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    if (--w03_depth == 0 && ld_lanes() > 1) ld_lane_join();
     return 0;
 }
 

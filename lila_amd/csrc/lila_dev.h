@@ -112,8 +112,18 @@ void ld_fill_normal(unsigned long long seed, int m, int n, double *A, int ld);
 
 /* ---- runtime ------------------------------------------------------------ */
 int  ld_init(void);               /* 0 on success */
-void *ld_stream(void);            /* hipStream_t */
-void ld_set_stream(void *s);
+void *ld_stream(void);            /* hipStream_t of the current lane */
+void ld_set_stream(void *s);      /* lane 0's stream (the library stream) */
+/* lanes: lane 0 runs on the library stream, lane 1 on an internal stream
+ * with its own workspace arena.  ld_lane_fork makes lane 1 wait for the
+ * work enqueued on lane 0 so far, ld_lane_join the reverse; ld_set_lane
+ * selects where the next ld_* calls go.  ld_lanes() == 1 keeps everything
+ * on lane 0. */
+int  ld_lanes(void);
+int  ld_set_lanes(int n);         /* returns the previous count */
+void ld_set_lane(int lane);
+void ld_lane_fork(void);
+void ld_lane_join(void);
 int  ld_is_device_ptr(const void *p);
 void *ld_ws_alloc(size_t bytes);  /* bump allocation from the arena */
 size_t ld_ws_mark(void);

@@ -30,11 +30,24 @@ struct Chunk {
   char *p;
   size_t cap, top;
 };
-struct State {
-  int inited = 0;
-  hipStream_t stream = nullptr;
+struct Arena {
   Chunk ch[MAXCHUNK];
   int nch = 0, cur = 0;
+};
+// Two lanes of work: lane 0 is the library stream (lila_set_stream), lane 1
+// an internal stream for work that may overlap lane 0's (the drivers fork
+// and join it with events).  Each lane bump-allocates from its own arena, so
+// the stream order of a lane is what keeps its workspace reuse safe.
+constexpr int NEV = 32;
+struct State {
+  int inited = 0;
+  hipStream_t stream = nullptr;       // the current lane's stream
+  hipStream_t user_stream = nullptr;  // lane 0
+  hipStream_t aux_stream = nullptr;   // lane 1
+  int lane = 0, lanes = 1;
+  Arena ar[2];
+  hipEvent_t ev[NEV];
+  int evi = 0;
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
@@ -59,7 +72,12 @@ int ld_init(void) {
     if (hipMalloc(&g.pscratch, pb) != hipSuccess) return -1;
     if (hipMemset(g.pscratch, 0, pb) != hipSuccess) return -1;
   }
-  const char *e = getenv("LILA_PANEL_ROWS");
+  if (hipStreamCreateWithFlags(&g.aux_stream, hipStreamNonBlocking) != hipSuccess) return -1;
+  for (int k = 0; k < NEV; k++)
+    if (hipEventCreateWithFlags(&g.ev[k], hipEventDisableTiming) != hipSuccess) return -1;
+  const char *e = getenv("LILA_LANES");
+  if (e) g.lanes = atoi(e) == 2 ? 2 : 1;
+  e = getenv("LILA_PANEL_ROWS");
   if (e) g.tune.panel_rows_per_wg = g.tune.panel_rows_cqr = atoi(e);
   e = getenv("LILA_GEMM_MIN_WGS");
   if (e) g.tune.gemm_min_wgs = atoi(e);
@@ -92,7 +110,40 @@ int ld_init(void) {
 ld_tune_t *ld_tune(void) { return &g.tune; }
 
 void *ld_stream(void) { return (void *)g.stream; }
-void ld_set_stream(void *s) { g.stream = (hipStream_t)s; }
+void ld_set_stream(void *s) {
+  g.user_stream = (hipStream_t)s;
+  if (g.lane == 0) g.stream = g.user_stream;
+}
+
+int ld_lanes(void) { return g.lanes; }
+int ld_set_lanes(int n) {
+  const int old = g.lanes;
+  g.lanes = (n == 1) ? 1 : 2;
+  return old;
+}
+void ld_set_lane(int l) {
+  g.lane = l ? 1 : 0;
+  g.stream = g.lane ? g.aux_stream : g.user_stream;
+}
+// lane 1 waits for everything lane 0 has enqueued so far
+void ld_lane_fork(void) {
+  hipEvent_t e = g.ev[g.evi];
+  g.evi = (g.evi + 1) % NEV;
+  HIPCHECK(hipEventRecord(e, g.user_stream));
+  HIPCHECK(hipStreamWaitEvent(g.aux_stream, e, 0));
+}
+static hipError_t sync_lanes(void) {
+  hipError_t e = hipStreamSynchronize(g.aux_stream);
+  hipError_t f = hipStreamSynchronize(g.user_stream);
+  return e != hipSuccess ? e : f;
+}
+// lane 0 waits for everything lane 1 has enqueued so far
+void ld_lane_join(void) {
+  hipEvent_t e = g.ev[g.evi];
+  g.evi = (g.evi + 1) % NEV;
+  HIPCHECK(hipEventRecord(e, g.aux_stream));
+  HIPCHECK(hipStreamWaitEvent(g.user_stream, e, 0));
+}
 
 int ld_is_device_ptr(const void *p) {
   if (!p) return 0;
@@ -105,8 +156,8 @@ int ld_is_device_ptr(const void *p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-static int ws_new_chunk(size_t bytes) {
-  if (g.nch >= MAXCHUNK) {
+static int ws_new_chunk(Arena &a, size_t bytes) {
+  if (a.nch >= MAXCHUNK) {
     fprintf(stderr, "lila_hip: workspace chunk list full\n");
     ld_set_info(-3);
     return -1;
@@ -117,68 +168,78 @@ static int ws_new_chunk(size_t bytes) {
     ld_set_info(-2);
     return -1;
   }
-  g.ch[g.nch].p = p;
-  g.ch[g.nch].cap = bytes;
-  g.ch[g.nch].top = 0;
-  g.nch++;
+  a.ch[a.nch].p = p;
+  a.ch[a.nch].cap = bytes;
+  a.ch[a.nch].top = 0;
+  a.nch++;
   return 0;
 }
 
 // make sure at least `bytes` are free in one chunk above the current top
 int ld_ws_reserve(size_t bytes) {
+  Arena &a = g.ar[g.lane];
   bytes = (bytes + 255) & ~(size_t)255;
-  if (g.nch > 0 && g.ch[g.cur].top + bytes <= g.ch[g.cur].cap) return 0;
-  for (int c = g.cur + 1; c < g.nch; c++)
-    if (g.ch[c].cap >= bytes) return 0;
+  if (a.nch > 0 && a.ch[a.cur].top + bytes <= a.ch[a.cur].cap) return 0;
+  for (int c = a.cur + 1; c < a.nch; c++)
+    if (a.ch[c].cap >= bytes) return 0;
   size_t want = bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes;
-  return ws_new_chunk(want);
+  return ws_new_chunk(a, want);
 }
 
 void *ld_ws_alloc(size_t bytes) {
+  Arena &a = g.ar[g.lane];
   bytes = (bytes + 255) & ~(size_t)255;
-  if (g.nch == 0 && ws_new_chunk(bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes) != 0) abort();
-  if (g.ch[g.cur].top + bytes > g.ch[g.cur].cap) {
+  if (a.nch == 0 && ws_new_chunk(a, bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes) != 0)
+    abort();
+  if (a.ch[a.cur].top + bytes > a.ch[a.cur].cap) {
     // move to the next chunk that fits (chunks above cur are empty)
-    int c = g.cur + 1;
-    while (c < g.nch && g.ch[c].cap < bytes) c++;
-    if (c >= g.nch) {
-      size_t want = 2 * g.ch[g.nch - 1].cap;
+    int c = a.cur + 1;
+    while (c < a.nch && a.ch[c].cap < bytes) c++;
+    if (c >= a.nch) {
+      size_t want = 2 * a.ch[a.nch - 1].cap;
       if (want < bytes) want = bytes;
-      if (ws_new_chunk(want) != 0) abort();
-      c = g.nch - 1;
+      if (ws_new_chunk(a, want) != 0) abort();
+      c = a.nch - 1;
     }
-    g.cur = c;
-    g.ch[c].top = 0;
+    a.cur = c;
+    a.ch[c].top = 0;
   }
-  void *p = g.ch[g.cur].p + g.ch[g.cur].top;
-  g.ch[g.cur].top += bytes;
+  void *p = a.ch[a.cur].p + a.ch[a.cur].top;
+  a.ch[a.cur].top += bytes;
   return p;
 }
 
 // mark = (chunk << 40) | top
 size_t ld_ws_mark(void) {
-  if (g.nch == 0) return 0;
-  return ((size_t)g.cur << 40) | g.ch[g.cur].top;
+  const Arena &a = g.ar[g.lane];
+  if (a.nch == 0) return 0;
+  return ((size_t)a.cur << 40) | a.ch[a.cur].top;
 }
 void ld_ws_release(size_t mark) {
-  if (g.nch == 0) return;
+  Arena &a = g.ar[g.lane];
+  if (a.nch == 0) return;
   int c = (int)(mark >> 40);
   size_t top = mark & (((size_t)1 << 40) - 1);
-  for (int k = c + 1; k < g.nch; k++) g.ch[k].top = 0;
-  g.cur = c;
-  g.ch[c].top = top;
+  for (int k = c + 1; k < a.nch; k++) a.ch[k].top = 0;
+  a.cur = c;
+  a.ch[c].top = top;
 }
 
-// after a synchronize, with nothing in use: fold several chunks into one
+// after a synchronize, with nothing in use: fold each lane's chunks into one
 void ld_ws_consolidate(void) {
-  if (g.nch <= 1) return;
-  size_t tot = 0;
-  for (int k = 0; k < g.nch; k++) tot += g.ch[k].cap;
-  HIPCHECK(hipStreamSynchronize(g.stream));
-  for (int k = 0; k < g.nch; k++) HIPCHECK(hipFree(g.ch[k].p));
-  g.nch = 0;
-  g.cur = 0;
-  ws_new_chunk(tot);
+  if (g.ar[0].nch <= 1 && g.ar[1].nch <= 1) return;
+  HIPCHECK(hipStreamSynchronize(g.user_stream));
+  HIPCHECK(hipStreamSynchronize(g.aux_stream));
+  for (int l = 0; l < 2; l++) {
+    Arena &a = g.ar[l];
+    if (a.nch <= 1) continue;
+    size_t tot = 0;
+    for (int k = 0; k < a.nch; k++) tot += a.ch[k].cap;
+    for (int k = 0; k < a.nch; k++) HIPCHECK(hipFree(a.ch[k].p));
+    a.nch = 0;
+    a.cur = 0;
+    ws_new_chunk(a, tot);
+  }
 }
 
 void *ld_dev_malloc(size_t bytes) {
@@ -226,7 +287,7 @@ int ld_check_device_flags(void) {
 }
 
 int ld_sync(void) {
-  hipError_t e = hipStreamSynchronize(g.stream);
+  hipError_t e = sync_lanes();
   if (e != hipSuccess) {
     fprintf(stderr, "lila_hip: stream error: %s\n", hipGetErrorString(e));
     ld_set_info(-1);
@@ -309,7 +370,7 @@ int ld_prof_begin(const char *name) {
     ncls++;
   }
   if (evused + 2 > 60000) {  // bound the outstanding events: drain
-    hipStreamSynchronize(g.stream);
+    sync_lanes();
     prof_flush();
   }
   if (nrec == caprec) {
@@ -340,6 +401,12 @@ int lila_set_panel_mode(int mode) {
   return old;
 }
 
+int lila_set_lanes(int n) {
+  if (ld_init() != 0) return -1;
+  sync_lanes();
+  return ld_set_lanes(n);
+}
+
 int lila_set_gemm_lowp(int on) {
   if (ld_init() != 0) return -1;
   const int old = g.tune.gemm_lowp;
@@ -349,7 +416,7 @@ int lila_set_gemm_lowp(int on) {
 
 int lila_prof_enable(int on) {
   if (ld_init() != 0) return -1;
-  hipStreamSynchronize(g.stream);
+  sync_lanes();
   prof_flush();
   if (on) ncls = 0;
   prof_on = on;
@@ -359,7 +426,7 @@ int lila_prof_enable(int on) {
 // restrict recording to one kernel class (NULL or "" = all classes)
 int lila_prof_only(const char *kernel_class) {
   if (ld_init() != 0) return -1;
-  hipStreamSynchronize(g.stream);
+  sync_lanes();
   prof_flush();
   snprintf(prof_only, sizeof prof_only, "%s", kernel_class ? kernel_class : "");
   return 0;
@@ -368,7 +435,7 @@ int lila_prof_only(const char *kernel_class) {
 // "name|launches|flops|bytes|ms\n" per kernel class; returns the class count
 int lila_prof_read(char *buf, int buflen) {
   if (ld_init() != 0) return -1;
-  hipStreamSynchronize(g.stream);
+  sync_lanes();
   prof_flush();
   int off = 0;
   for (int c = 0; c < ncls && off < buflen; c++)
