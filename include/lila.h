@@ -341,6 +341,20 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
  * (dgeqr3's arithmetic).  Both give dgeqr3's V, R, T and Q to rounding. */
 int lila_set_panel_mode(int mode);
 
+/* Cholesky-QR of the hr paths (lila_dgeqrf_w03_mt_hr, lila_dgeqr2_w03_hr
+ * and the hr leaves; returns the previous setting, -1 outside 0..3): 0
+ * (default) one pass with an FP64 Gram, as the reference; 1, 2, 3 two passes
+ * (Cholesky-QR2) whose first Gram is FP64 MFMA, FP32 MFMA (operands rounded
+ * to FP32) or 3-term split-BF16 MFMA with FP32 accumulation (n <= 256), the
+ * second FP64.  A Cholesky breakdown (the first Gram not positive definite
+ * at its precision) returns the column as a positive info. */
+int lila_set_gram_precision(int prec);
+
+/* G (n x n; only the upper triangle is written) = A^T A of the m x n A in
+ * the arithmetic of prec: 0 or 1 FP64 MFMA, 2 FP32 MFMA, 3 split BF16
+ * (n <= 256).  The Gram of the Cholesky-QR paths, exposed for measurement. */
+int lila_dgram(int prec, int m, int n, double *A, int lda, double *G, int ldg);
+
 /* Streams of lila_dgeqrf_w03_recursive (returns the previous count): 1
  * (default) keeps every step on the library stream; 2 builds Q (the
  * dormqrbz steps) on a second, internal stream, overlapped with the

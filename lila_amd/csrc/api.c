@@ -630,6 +630,17 @@ int qr2_aux_dlarf_wrapper(char side, int m, int n, double *V, int incv, double t
     return api_end();
 }
 
+/* G (n x n, upper triangle written) = A^T A for the m x n A, in the
+ * arithmetic of prec (0 or 1 FP64 MFMA, 2 FP32 MFMA, 3 split BF16) */
+int lila_dgram(int prec, int m, int n, double *A, int lda, double *G, int ldg) {
+    if (prec < 0 || prec > 3 || m < 0 || n < 0) return -1;
+    BEGIN();
+    stage_t a = stage_in(A, lda, m, n), g = stage_in(G, ldg, n, n);
+    dv_gram(prec, m, n, a.dev, lda, g.dev, ldg);
+    stage_out(&a, 0); stage_out(&g, 1);
+    return api_end();
+}
+
 /* ---- least squares, generators, status --------------------------------- */
 
 int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb) {

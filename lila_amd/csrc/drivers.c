@@ -264,11 +264,39 @@ int dv_dorghr(int m, int n, int i, int mt, double *A, int lda, double *T, int ld
 }
 
 /* one Cholesky-QR pass: Q = A R^{-1}, R = chol(A^T A) in the top of A */
+/* G (upper) = X^T X in the arithmetic of `prec`: FP64 MFMA (0, 1), FP32
+ * MFMA with operands rounded to FP32 (2), 3-term split-BF16 MFMA with FP32
+ * accumulation (3, n <= 256; FP32 MFMA above) */
+int dv_gram(int prec, int ml, int n, const double *X, int ldx, double *G, int ldg) {
+    if (prec == 3 && n <= 256) return ld_gram_bf16x3(ml, n, X, ldx, G, ldg);
+    ld_tune_t *tu = ld_tune();
+    const int old = tu->gemm_lowp;
+    if (prec >= 2) tu->gemm_lowp = 1;
+    ld_syrk_ut(n, ml, 1.0, X, ldx, 0.0, G, ldg);
+    tu->gemm_lowp = old;
+    return 0;
+}
+
+/* Cholesky-QR of the hr paths (lila-c/lila_dgeqrf_w03_mt_hr.c:18-21: dsyrk,
+ * dpotrf, dtrsm): Q = A R^-1 with R on and above A's diagonal.  With the
+ * Gram precision set to 1, 2 or 3 it runs twice (Cholesky-QR2), the first
+ * Gram in FP64, FP32 MFMA or split BF16 and the second in FP64:
+ * Q = (A R1^-1) R2^-1, R = R2 R1. */
 static void cholqr(int ml, int n, double *Aii, int lda, double *Qii, int ldq) {
+    const int prec = ld_tune()->gram_prec;
     ld_lacpy('A', ml, n, Aii, lda, Qii, ldq);
-    ld_syrk_ut(n, ml, 1.0, Qii, ldq, 0.0, Aii, lda);
+    dv_gram(prec, ml, n, Qii, ldq, Aii, lda);
     ld_potrf_u(n, Aii, lda);
     ld_trsm('R', 'U', 'N', 'N', ml, n, 1.0, Aii, lda, Qii, ldq);
+    if (prec == 0) return;
+    size_t mark = ld_ws_mark();
+    double *G = wsd((size_t)n * n), *R = wsd((size_t)n * n);
+    ld_syrk_ut(n, ml, 1.0, Qii, ldq, 0.0, G, n);
+    ld_potrf_u(n, G, n);
+    ld_trsm('R', 'U', 'N', 'N', ml, n, 1.0, G, n, Qii, ldq);
+    gemm(N_, N_, n, n, n, 1.0, G, n, LD_UPPER, Aii, lda, LD_UPPER, 0.0, R, n);
+    ld_lacpy('U', n, n, R, n, Aii, lda);
+    ld_ws_release(mark);
 }
 
 /* CholQR leaf + blocked reconstruction; lila-c/lila_dgeqrf_w03_mt_hr.c:3-33 */

@@ -82,6 +82,9 @@ void ld_trsm(char side, char uplo, char trans, char diag, int m, int n,
 /* Triangular multiply B = alpha op(A) B or alpha B op(A), in place. */
 void ld_trmm(char side, char uplo, char trans, char diag, int m, int n,
              double alpha, const double *A, int lda, double *B, int ldb);
+/* G(upper) = A^T A for n <= 256 with 3-term split-BF16 MFMA products and
+ * FP32 accumulation (gram.hip); -1 when n > 256 */
+int ld_gram_bf16x3(int m, int n, const double *A, int lda, double *G, int ldg);
 /* C(upper) = alpha A^T A + beta C(upper)  (A is k x n) */
 void ld_syrk_ut(int n, int k, double alpha, const double *A, int lda,
                 double beta, double *C, int ldc);
@@ -159,6 +162,8 @@ typedef struct {
     int panel_mode;          /* 0 column Householder, 1 Cholesky-QR2 + reconstruction */
     int gemm_lowp;           /* 1: GEMMs on FP32 MFMA (operands rounded to FP32) */
     int gemm_bm_t, gemm_bn_t; /* tile of products with op(A) = A^T (W = V^T C) */
+    int gram_prec;           /* Cholesky-QR of the hr paths: 0 one FP64 pass; 1, 2, 3 two
+                                passes whose first Gram is FP64, FP32 MFMA, split BF16 */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

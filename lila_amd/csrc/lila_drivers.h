@@ -96,6 +96,7 @@ int dq_dgeqrf_w03_levelx(const int *p, int n_lvl, int i_lvl, const int *nb_lvl, 
 int dv_qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
 int dv_qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau);
 int dv_qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau);
+int dv_gram(int prec, int ml, int n, const double *X, int ldx, double *G, int ldg);
 int dv_qr2_dgeqr3_R_UT(int m, int n, double *A, int lda, double *T, int ldt, double *R, int ldr);
 int dv_qr2_dgeqr3_R_isw(int ut, int m, int n, double *A, int lda, double *T, int ldt, double *R,
                         int ldr);

@@ -51,7 +51,7 @@ struct State {
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
-  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128};
+  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0};
 };
 State g;
 }  // namespace
@@ -398,6 +398,14 @@ int lila_set_panel_mode(int mode) {
   if (ld_init() != 0) return -1;
   const int old = g.tune.panel_mode;
   g.tune.panel_mode = mode ? 1 : 0;
+  return old;
+}
+
+int lila_set_gram_precision(int prec) {
+  if (ld_init() != 0) return -1;
+  if (prec < 0 || prec > 3) return -1;
+  const int old = g.tune.gram_prec;
+  g.tune.gram_prec = prec;
   return old;
 }
 
