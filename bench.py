@@ -26,6 +26,11 @@ Besides the JSON line's contract fields it reports:
                   n ~ U{8..min(128,m)}, nrhs ~ U{1..16}) solved by one
                   persistent launch replayed from a hipGraph: QR/s, and the
                   oracle's per-problem dgels on the host cores beside it;
+  tall_cholqr  -- the tall-skinny Cholesky-QR path at m = 1e6, n = 256: the
+                  Gram A^T A in FP64, FP32 and 3-term split-BF16 MFMA
+                  arithmetic (GB/s against the 8 TB/s HBM roofline of one
+                  read of A), and lila_dgeqrf_w03_mt_hr with each Gram
+                  precision (time, orthogonality, backward error);
   cpu_baseline -- the CPU oracle (a C restatement of LILA's w03 recursion,
                   BLAS-3 steps through the host OpenBLAS, all its threads) on
                   a bounded sample of the same workload, rank 0 only;
@@ -275,6 +280,85 @@ def batch_cpu_baseline(m, n, k, budget_s, seed):
     }
 
 
+def tall_leg(lila_amd, lib, torch, steps, m=1_000_000, n=256):
+    """Tall-skinny Cholesky-QR path at m = 1e6, n = 256 (N(0,1) entries,
+    resident in HBM): the Gram A^T A in each arithmetic (FP64 MFMA, FP32
+    MFMA, 3-term split-BF16 MFMA; one read of A, so the HBM roofline is
+    8 m n bytes / 8 TB/s), then lila_dgeqrf_w03_mt_hr with each Gram precision
+    (Cholesky-QR once, or twice with that first Gram, then the Householder
+    reconstruction of V, T and Q).  Times: events on the library stream
+    around the calls only."""
+    f64 = torch.float64
+    A0 = torch.empty((n, m), dtype=f64, device="cuda")
+    lila_amd.call("lila_fill_normal", ctypes.c_ulonglong(20241205), m, n, A0, m)
+    G = torch.zeros((n, n), dtype=f64, device="cuda")
+    reps = max(3, steps)
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    names = {1: "fp64_mfma", 2: "fp32_mfma", 3: "bf16x3_mfma"}
+    gram, Gs = {}, {}
+    nbytes = 8.0 * m * n
+    for prec in (1, 2, 3):
+        lila_amd.call("lila_dgram", prec, m, n, A0, m, G, n)
+        torch.cuda.synchronize()
+        e0, e1 = ev(), ev()
+        e0.record()
+        for _ in range(reps):
+            lila_amd.call("lila_dgram", prec, m, n, A0, m, G, n)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        Gs[prec] = torch.triu(G).clone()
+        gram[names[prec]] = {"ms": ms, "GB/s": nbytes / (ms * 1e-3) / 1e9,
+                             "frac_hbm": nbytes / (ms * 1e-3) / HBM_PEAK / 1e9,
+                             "TFLOP/s": m * n * (n + 1) / (ms * 1e-3) / 1e12}
+    for prec in (2, 3):
+        gram[names[prec]]["rel_err_vs_fp64"] = (torch.linalg.norm(Gs[prec] - Gs[1]) /
+                                                torch.linalg.norm(Gs[1])).item()
+    # the split-BF16 kernel alone (without the split-K reduction), from the
+    # launch profile
+    lib.lila_prof_enable(1)
+    lila_amd.call("lila_dgram", 3, m, n, A0, m, G, n)
+    torch.cuda.synchronize()
+    rows = {r["name"]: r for r in read_prof(lib)}
+    lib.lila_prof_enable(0)
+    if "gram3_kernel" in rows:
+        k = rows["gram3_kernel"]
+        kms = k["ms"] / k["launches"]
+        gram["bf16x3_mfma"]["kernel_ms"] = kms
+        gram["bf16x3_mfma"]["kernel_frac_hbm"] = nbytes / (kms * 1e-3) / HBM_PEAK / 1e9
+    A = torch.empty_like(A0)
+    Q = torch.empty_like(A0)
+    T = torch.zeros((n, n), dtype=f64, device="cuda")
+    nA = torch.linalg.norm(A0).item()
+    path = {}
+    for prec in (0, 1, 2, 3):
+        lib.lila_set_gram_precision(prec)
+        tot = 0.0
+        for r in range(reps + 1):
+            A.copy_(A0)
+            torch.cuda.synchronize()
+            e0, e1 = ev(), ev()
+            e0.record()
+            rc = lila_amd.call("lila_dgeqrf_w03_mt_hr", m, n, 0, n, A, m, T, n, Q, m, None, 0)
+            e1.record()
+            torch.cuda.synchronize()
+            if r > 0:
+                tot += e0.elapsed_time(e1)
+        info = lila_amd.call("lila_get_info")
+        Qm = Q.T
+        orth = torch.linalg.norm(Qm.T @ Qm - torch.eye(n, dtype=f64, device="cuda")).item()
+        bwd = torch.linalg.norm(A0.T - Qm @ torch.triu(A.T[:n])).item() / nA
+        path[["one_pass_fp64", "two_pass_fp64", "two_pass_fp32", "two_pass_bf16x3"][prec]] = {
+            "ms": tot / reps, "return": rc, "info": info, "orthogonality": orth,
+            "backward_error": bwd}
+    lib.lila_set_gram_precision(0)
+    del A0, A, Q
+    return {"workload": "lila_dgeqrf_w03_mt_hr (Cholesky-QR + Householder reconstruction) and "
+                        "its Gram, m=%d, n=%d, N(0,1)" % (m, n),
+            "gram": gram, "cholqr_path": path,
+            "accuracy_per_kappa": "tests/test_gpu_gram.py (kappa 1e1, 1e3, 1e6)"}
+
+
 def batch_leg(lila_amd, lib, torch, dev, steps, warmup, cpu_seconds, no_cpu, nprob=4096):
     """4096 uneven least-squares problems (m ~ U{64..4096}, n ~ U{8..128},
     m >= n, nrhs ~ U{1..16}, N(0,1) entries) in one persistent launch replayed
@@ -388,6 +472,9 @@ def main():
                     help="1: Cholesky-QR2 + reconstruction panel (default), 0: column Householder")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched least-squares leg")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched least-squares leg")
+    ap.add_argument("--no-tall", action="store_true",
+                    help="skip the tall-skinny Cholesky-QR leg (m=1e6, n=256)")
+    ap.add_argument("--tall-only", action="store_true", help="run only the tall-skinny leg")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision leg and its per-kappa accuracy table")
     ap.add_argument("--autotune", action="store_true",
@@ -409,6 +496,9 @@ def main():
     if args.batch_only:
         print(json.dumps(batch_leg(lila_amd, lib, torch, dev, args.steps, args.warmup,
                                    args.cpu_seconds, args.no_cpu)))
+        return
+    if args.tall_only:
+        print(json.dumps(tall_leg(lila_amd, lib, torch, args.steps)))
         return
     m, n, mt, nx = args.m, args.n, args.mt, args.nx
     tuned = None
@@ -487,6 +577,10 @@ def main():
     if not args.no_mixed:
         mixed = mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, args.steps, bwd, orth, dt)
 
+    tall = None
+    if not args.no_tall:
+        tall = tall_leg(lila_amd, lib, torch, args.steps)
+
     batch = None
     if not args.no_batch:
         batch = batch_leg(lila_amd, lib, torch, dev, args.steps, args.warmup, args.cpu_seconds,
@@ -537,6 +631,7 @@ def main():
         "accuracy": {"backward_error": bwd, "orthogonality": orth, "device_status": info},
         "mixed": mixed,
         "batch": batch,
+        "tall_cholqr": tall,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_classes": prof_all,

@@ -264,11 +264,13 @@ int dv_dorghr(int m, int n, int i, int mt, double *A, int lda, double *T, int ld
 }
 
 /* one Cholesky-QR pass: Q = A R^{-1}, R = chol(A^T A) in the top of A */
-/* G (upper) = X^T X in the arithmetic of `prec`: FP64 MFMA (0, 1), FP32
+/* G (upper) = X^T X in the arithmetic of `prec`: FP64 MFMA (0, 1; the
+ * tall-Gram kernel for n <= 256, the SYRK form of the GEMM above), FP32
  * MFMA with operands rounded to FP32 (2), 3-term split-BF16 MFMA with FP32
  * accumulation (3, n <= 256; FP32 MFMA above) */
 int dv_gram(int prec, int ml, int n, const double *X, int ldx, double *G, int ldg) {
     if (prec == 3 && n <= 256) return ld_gram_bf16x3(ml, n, X, ldx, G, ldg);
+    if (prec <= 1 && n <= 256) return ld_gram_f64(ml, n, X, ldx, G, ldg);
     ld_tune_t *tu = ld_tune();
     const int old = tu->gemm_lowp;
     if (prec >= 2) tu->gemm_lowp = 1;
@@ -291,7 +293,7 @@ static void cholqr(int ml, int n, double *Aii, int lda, double *Qii, int ldq) {
     if (prec == 0) return;
     size_t mark = ld_ws_mark();
     double *G = wsd((size_t)n * n), *R = wsd((size_t)n * n);
-    ld_syrk_ut(n, ml, 1.0, Qii, ldq, 0.0, G, n);
+    dv_gram(1, ml, n, Qii, ldq, G, n);
     ld_potrf_u(n, G, n);
     ld_trsm('R', 'U', 'N', 'N', ml, n, 1.0, G, n, Qii, ldq);
     gemm(N_, N_, n, n, n, 1.0, G, n, LD_UPPER, Aii, lda, LD_UPPER, 0.0, R, n);

@@ -85,6 +85,8 @@ void ld_trmm(char side, char uplo, char trans, char diag, int m, int n,
 /* G(upper) = A^T A for n <= 256 with 3-term split-BF16 MFMA products and
  * FP32 accumulation (gram.hip); -1 when n > 256 */
 int ld_gram_bf16x3(int m, int n, const double *A, int lda, double *G, int ldg);
+/* the same in FP64 MFMA arithmetic */
+int ld_gram_f64(int m, int n, const double *A, int lda, double *G, int ldg);
 /* C(upper) = alpha A^T A + beta C(upper)  (A is k x n) */
 void ld_syrk_ut(int n, int k, double alpha, const double *A, int lda,
                 double beta, double *C, int ldc);
