@@ -31,6 +31,8 @@ Besides the JSON line's contract fields it reports:
                   arithmetic (GB/s against the 8 TB/s HBM roofline of one
                   read of A), and lila_dgeqrf_w03_mt_hr with each Gram
                   precision (time, orthogonality, backward error);
+  least_squares -- one m = 100000, n = 1024, nrhs = 64 least-squares problem
+                  through lila_dgels_qr3, with host LAPACK dgels beside it;
   cpu_baseline -- the CPU oracle (a C restatement of LILA's w03 recursion,
                   BLAS-3 steps through the host OpenBLAS, all its threads) on
                   a bounded sample of the same workload, rank 0 only;
@@ -280,6 +282,58 @@ def batch_cpu_baseline(m, n, k, budget_s, seed):
     }
 
 
+def ls_leg(lila_amd, lib, torch, steps, no_cpu, m=100_000, n=1024, k=64):
+    """One large least-squares problem through lila_dgels_qr3 (the sequence of
+    least-squares/main_ls_qr3.c:94-119: dgeqr3 with T, Q^T B, R X = (Q^T B)_1),
+    N(0,1) A (m x n) and B (m x k) resident in HBM, restored from pristine
+    copies outside the timed calls; rate by the count of
+    least-squares/main_ls_qr3.c:125-129.  Beside it, host LAPACK dgels (the
+    reference's comparison, least-squares/main_lapack_ref.c:100) on the same
+    problem, all host threads."""
+    f64 = torch.float64
+    A0 = torch.empty((n, m), dtype=f64, device="cuda")
+    B0 = torch.empty((k, m), dtype=f64, device="cuda")
+    lila_amd.call("lila_fill_normal", ctypes.c_ulonglong(20241206), m, n, A0, m)
+    lila_amd.call("lila_fill_normal", ctypes.c_ulonglong(20241207), m, k, B0, m)
+    A, B = torch.empty_like(A0), torch.empty_like(B0)
+    T = torch.zeros((n, n), dtype=f64, device="cuda")
+    reps = max(3, steps)
+    tot = 0.0
+    for r in range(reps + 1):
+        A.copy_(A0)
+        B.copy_(B0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = lila_amd.call("lila_dgels_qr3", m, n, k, A, m, T, n, B, m)
+        e1.record()
+        torch.cuda.synchronize()
+        if r > 0:
+            tot += e0.elapsed_time(e1)
+    ms = tot / reps
+    flops = float(ls_flops(m, n, k))
+    X = B[:, :n].T  # n x k
+    Am, Bm = A0.T, B0.T
+    res = Bm - Am @ X
+    nA = torch.linalg.norm(Am).item()
+    eta = (torch.linalg.norm(Am.T @ res) /
+           (nA * (nA * torch.linalg.norm(X) + torch.linalg.norm(Bm)))).item()
+    out = {"workload": "lila_dgels_qr3 m=%d n=%d nrhs=%d N(0,1)" % (m, n, k),
+           "ms": ms, "TFLOP/s": flops / (ms * 1e-3) / 1e12, "return": rc,
+           "normal_equation_residual": eta}
+    if not no_cpu:
+        import scipy.linalg.lapack as sl
+        Ah = np.asfortranarray(A0.T.cpu().numpy())
+        Bh = np.asfortranarray(B0.T.cpu().numpy())
+        t0 = time.perf_counter()
+        sl.dgels(Ah, Bh, overwrite_a=True, overwrite_b=True)
+        tc = time.perf_counter() - t0
+        out["cpu_lapack_dgels"] = {"s": tc, "TFLOP/s": flops / tc / 1e12,
+                                   "threads": "host LAPACK default (all host threads)"}
+    del A0, B0, A, B
+    return out
+
+
 def tall_leg(lila_amd, lib, torch, steps, m=1_000_000, n=256):
     """Tall-skinny Cholesky-QR path at m = 1e6, n = 256 (N(0,1) entries,
     resident in HBM): the Gram A^T A in each arithmetic (FP64 MFMA, FP32
@@ -475,6 +529,8 @@ def main():
     ap.add_argument("--no-tall", action="store_true",
                     help="skip the tall-skinny Cholesky-QR leg (m=1e6, n=256)")
     ap.add_argument("--tall-only", action="store_true", help="run only the tall-skinny leg")
+    ap.add_argument("--no-ls", action="store_true",
+                    help="skip the large least-squares leg (m=100000, n=1024, 64 right-hand sides)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision leg and its per-kappa accuracy table")
     ap.add_argument("--autotune", action="store_true",
@@ -577,6 +633,10 @@ def main():
     if not args.no_mixed:
         mixed = mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, args.steps, bwd, orth, dt)
 
+    ls = None
+    if not args.no_ls:
+        ls = ls_leg(lila_amd, lib, torch, args.steps, args.no_cpu)
+
     tall = None
     if not args.no_tall:
         tall = tall_leg(lila_amd, lib, torch, args.steps)
@@ -632,6 +692,7 @@ def main():
         "mixed": mixed,
         "batch": batch,
         "tall_cholqr": tall,
+        "least_squares": ls,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_classes": prof_all,
