@@ -271,6 +271,41 @@ int qr2_aux_dUinvLTinU(int n, double *L, int ldl, double *U, int ldu);
 int qr2_aux_dlarf_wrapper(char side, int m, int n, double *V, int incv, double tau, double *C,
                           int ldc, double *work);
 
+/* ---- operation counts of the recursive QR library (qr3/flops/flops.h) -----
+ * Host arithmetic (lila_amd/csrc/flops.c), the reference's formulas: a
+ * closed form per routine and, for the recursive ones, the `_check` sum over
+ * the recursion's steps.  qr3/flops/flops.h:7-10 */
+long int flops_gemm(int m, int n, int k);
+long int flops_trmm(char S, int m, int n);
+long int flops_syrk(int n, int k);
+long int flops_larft(int m, int k);
+/* qr3/flops/flops.h:13-25 */
+long int flops_mLUinA(int n);
+long int flops_V2N(int n);
+long int flops_ApUBTinA(int m, int n);
+long int flops_ULTinU(int n);
+long int flops_N2T(int n);
+long int flops_VT2Q(int m, int n);
+long int flops_mLUinA_check(int n);
+long int flops_V2N_check(int n);
+long int flops_ApUBTinA_check(int m, int n);
+long int flops_ULTinU_check(int n);
+long int flops_N2T_check(int n);
+long int flops_VT2Q_check(int m, int n);
+/* qr3/flops/flops.h:30-41 */
+long int flops_lapack_larf(int m, int n);
+long int flops_lapack_larfg(int m);
+long int flops_lapack_geqr2(int m, int n);
+long int flops_lapack_org2r_check(int m, int n, int k);
+long int flops_lapack_org2r(int m, int n, int k);
+/* qr3/flops/flops.h:62-63, 87-91 */
+long int flops_geqr3(int m, int n);
+long int flops_geqr3_check(int m, int n);
+long int flops_qr2_dorgqr_check(int m, int n);
+long int flops_qr2_dorgqr(int m, int n);
+long int flops_dlarft3_check(int m, int n);
+long int flops_dlarft3(int m, int n);
+
 /* ---- extensions (no counterpart in the reference header) ------------- */
 
 /* Status of the last call (0 ok, >0 numerical info, <0 error), after

@@ -80,7 +80,8 @@ def exported_names():
     import re
     txt = open(INCLUDE).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void|double)[\s*]+(\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|long int)[\s*]+(\w+)\s*\(", txt,
+                                 flags=re.M)))
 
 
 def autotune(m, n, reps=1, mt=512, nx=32):
