@@ -28,15 +28,33 @@ static void gemm(int ta, int tb, int m, int n, int k, double alpha, const double
 
 /* ---- block reflector pieces ------------------------------------------- */
 
+/* W = op(T) (op(A) op(B)) for a small T (op(T) = T^T when transT): one
+ * split product plus one launch that sums it and multiplies by op(T) when
+ * the product is split over K, else the product and a triangular GEMM */
+static void gemm_then_tri(int ta, int tb, int m, int n, int k, const double *A, int lda, int sa,
+                          const double *B, int ldb, int sb, const double *T, int ldt, int transT,
+                          double *W, int ldw) {
+    ld_gemm_t g;
+    memset(&g, 0, sizeof g);
+    g.m = m; g.n = n; g.k = k; g.alpha = 1.0; g.beta = 0.0;
+    g.A = A; g.lda = lda; g.ta = ta; g.sa = sa;
+    g.B = B; g.ldb = ldb; g.tb = tb; g.sb = sb;
+    if (ld_gemm_tmul(&g, T, ldt, transT, W, ldw) == 0) return;
+    size_t mark = ld_ws_mark();
+    double *G = wsd((size_t)m * n);
+    gemm(ta, tb, m, n, k, 1.0, A, lda, sa, B, ldb, sb, 0.0, G, m);
+    gemm(transT ? T_ : N_, N_, m, n, m, 1.0, T, ldt, LD_UPPER, G, m, LD_GEN, 0.0, W, ldw);
+    ld_ws_release(mark);
+}
+
 /* C (rows [0, ml) of the block) <- (I - V T V^T)^T C for one block of vb
  * reflectors: W = V^T C; W = T^T W; C -= V W. */
 static void apply_block_qt(int ml, int n, int vb, const double *V, int ldv, const double *Tb,
                            int ldt, double *C, int ldc) {
     if (vb <= 0 || n <= 0) return;
     size_t mark = ld_ws_mark();
-    double *W1 = wsd((size_t)vb * n), *W2 = wsd((size_t)vb * n);
-    gemm(T_, N_, vb, n, ml, 1.0, V, ldv, LD_VUNIT, C, ldc, LD_GEN, 0.0, W1, vb);
-    gemm(T_, N_, vb, n, vb, 1.0, Tb, ldt, LD_UPPER, W1, vb, LD_GEN, 0.0, W2, vb);
+    double *W2 = wsd((size_t)vb * n);
+    gemm_then_tri(T_, N_, vb, n, ml, V, ldv, LD_VUNIT, C, ldc, LD_GEN, Tb, ldt, 1, W2, vb);
     gemm(N_, N_, ml, n, vb, -1.0, V, ldv, LD_VUNIT, W2, vb, LD_GEN, 1.0, C, ldc);
     ld_ws_release(mark);
 }
@@ -94,9 +112,9 @@ int dv_dormqrbz_w03(int m, int n, int k, int i, int j, int mt, double *A, int ld
         const double *Tb = T + s % mt + (size_t)s * ldt;
         double *Qb = Q + s + (size_t)j * ldq;
         size_t mark = ld_ws_mark();
-        double *W1 = wsd((size_t)w * n), *W2 = wsd((size_t)w * n);
-        gemm(T_, N_, w, n, ml - w, 1.0, Vb + w, lda, LD_GEN, Qb + w, ldq, LD_GEN, 0.0, W1, w);
-        gemm(N_, N_, w, n, w, 1.0, Tb, ldt, LD_UPPER, W1, w, LD_GEN, 0.0, W2, w);
+        double *W2 = wsd((size_t)w * n);
+        gemm_then_tri(T_, N_, w, n, ml - w, Vb + w, lda, LD_GEN, Qb + w, ldq, LD_GEN, Tb, ldt, 0, W2,
+                      w);
         ld_gemm_t g;
         memset(&g, 0, sizeof g);
         g.m = ml; g.n = n; g.k = w; g.alpha = -1.0; g.beta = 1.0;
@@ -126,9 +144,8 @@ int dv_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int lda
     double *Tji = T + (i % mt - wb) + (size_t)i * ldt;
     const double *Tjj = T + (i % mt - wb) + (size_t)(i - wb) * ldt;
     size_t mark = ld_ws_mark();
-    double *G = wsd((size_t)wb * vb), *H = wsd((size_t)wb * vb);
-    gemm(T_, N_, wb, vb, m - i, 1.0, Aij, lda, LD_GEN, Aii, lda, LD_VUNIT, 0.0, G, wb);
-    gemm(N_, N_, wb, vb, wb, 1.0, Tjj, ldt, LD_UPPER, G, wb, LD_GEN, 0.0, H, wb);
+    double *H = wsd((size_t)wb * vb);
+    gemm_then_tri(T_, N_, wb, vb, m - i, Aij, lda, LD_GEN, Aii, lda, LD_VUNIT, Tjj, ldt, 0, H, wb);
     gemm(N_, N_, wb, vb, vb, -1.0, H, wb, LD_GEN, Tki, ldt, LD_UPPER, 0.0, Tji, ldt);
     ld_ws_release(mark);
     return 0;
@@ -151,10 +168,9 @@ int dv_dgeqr3(int m, int n, double *A, int lda, double *T, int ldt) {
     apply_block_qt(m, n2, n1, A, lda, T, ldt, A + (size_t)n1 * lda, lda);
     dv_dgeqr3(m - n1, n2, A + n1 + (size_t)n1 * lda, lda, T + n1 + (size_t)n1 * ldt, ldt);
     size_t mark = ld_ws_mark();
-    double *G = wsd((size_t)n1 * n2), *H = wsd((size_t)n1 * n2);
-    gemm(T_, N_, n1, n2, m - n1, 1.0, A + n1, lda, LD_GEN, A + n1 + (size_t)n1 * lda, lda, LD_VUNIT,
-         0.0, G, n1);
-    gemm(N_, N_, n1, n2, n1, 1.0, T, ldt, LD_UPPER, G, n1, LD_GEN, 0.0, H, n1);
+    double *H = wsd((size_t)n1 * n2);
+    gemm_then_tri(T_, N_, n1, n2, m - n1, A + n1, lda, LD_GEN, A + n1 + (size_t)n1 * lda, lda,
+                  LD_VUNIT, T, ldt, 0, H, n1);
     gemm(N_, N_, n1, n2, n2, -1.0, H, n1, LD_GEN, T + n1 + (size_t)n1 * ldt, ldt, LD_UPPER, 0.0,
          T + (size_t)n1 * ldt, ldt);
     ld_ws_release(mark);

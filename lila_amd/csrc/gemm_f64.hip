@@ -297,6 +297,62 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int S,
   *c = o;
 }
 
+// W(:, j) = alpha op(T) sum_z P_z(:, j) for a split product of m <= MAXM
+// rows (op(T) = T^T when transT, else T; T upper): one workgroup per column
+// j.  Thread t sums partials z = q, q + Q, ... of row i (i = t % Mp,
+// q = t / Mp, Mp = 32, 64 or 128, Q = 256 / Mp) with four accumulators; the Q sums are
+// combined in a fixed order, so the result does not depend on timing.
+// op(T) is staged in LDS.
+template <int MAXM>
+__global__ __launch_bounds__(256) void splitk_reduce_tmul_kernel(int M, int N, int S, double alpha,
+                                                                 const double *__restrict__ P,
+                                                                 const double *__restrict__ T,
+                                                                 int ldt, int transT,
+                                                                 double *__restrict__ W, int ldw) {
+  __shared__ double red[256];
+  __shared__ double g[MAXM];
+  __shared__ double Ts[MAXM][MAXM + 1];
+  const int j = blockIdx.x, t = threadIdx.x;
+  const int Mp = (M <= 32) ? 32 : (M <= 64) ? 64 : 128, Q = 256 / Mp;
+  const int i = t % Mp, q = t / Mp;
+  const size_t tot = (size_t)M * N;
+  for (int e = t; e < M * M; e += 256) {
+    const int r = e % M, c = e / M;
+    double v = 0.0;
+    if (transT) {
+      if (c <= r) v = T[(size_t)c + (size_t)r * ldt];
+    } else if (c >= r) {
+      v = T[(size_t)r + (size_t)c * ldt];
+    }
+    Ts[r][c] = v;
+  }
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (i < M) {
+    const double *p = P + i + (size_t)j * M;
+    int z = q;
+    for (; z + 3 * Q < S; z += 4 * Q) {
+      s0 += p[(size_t)z * tot];
+      s1 += p[(size_t)(z + Q) * tot];
+      s2 += p[(size_t)(z + 2 * Q) * tot];
+      s3 += p[(size_t)(z + 3 * Q) * tot];
+    }
+    for (; z < S; z += Q) s0 += p[(size_t)z * tot];
+  }
+  red[q * Mp + i] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (t < M) {
+    double s = red[t];
+    for (int r = 1; r < Q; r++) s += red[r * Mp + t];
+    g[t] = s;
+  }
+  __syncthreads();
+  if (t < M) {
+    double w = 0.0;
+    for (int l = 0; l < M; l++) w += Ts[t][l] * g[l];
+    W[(size_t)t + (size_t)j * ldw] = alpha * w;
+  }
+}
+
 template <int BM, int BN>
 void launch_tile(const ld_gemm_t *g, int splits, int kchunk, double *P, hipStream_t st) {
   dim3 grid((g->m + BM - 1) / BM, (g->n + BN - 1) / BN, splits);
@@ -329,19 +385,12 @@ void launch_tile(const ld_gemm_t *g, int splits, int kchunk, double *P, hipStrea
 
 }  // namespace
 
-extern "C" void ld_gemm(const ld_gemm_t *g) {
-  if (g->m <= 0 || g->n <= 0) return;
-  hipStream_t st = (hipStream_t)ld_stream();
-  if (g->k <= 0 || g->alpha == 0.0) {
-    // C = beta C (rows < zero_rows -> 0)
-    ld_gemm_t h = *g;
-    h.k = 0;
-    launch_tile<64, 64>(&h, 1, 0, nullptr, st);
-    return;
-  }
+// tile shape and split-K count of a product
+static void gemm_plan(const ld_gemm_t *g, int &bm, int &bn, int &splits, int &kchunk) {
   const int M = g->m, N = g->n, K = g->k;
   const ld_tune_t *tu = ld_tune();
-  int bm = g->ta ? tu->gemm_bm_t : tu->gemm_bm, bn = g->ta ? tu->gemm_bn_t : tu->gemm_bn;
+  bm = g->ta ? tu->gemm_bm_t : tu->gemm_bm;
+  bn = g->ta ? tu->gemm_bn_t : tu->gemm_bn;
   if (M <= 32) bm = 32; else if (M <= 64 && bm > 64) bm = 64;
   if (N <= 32) bn = 32; else if (N <= 64 && bn > 64) bn = 64;
   long tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
@@ -355,7 +404,7 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
     bm = 64; bn = 64;
     tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
   }
-  int splits = 1;
+  splits = 1;
   const int target = tu->gemm_min_wgs;
   if (tiles < target && K >= 256) {
     splits = (int)((target + tiles - 1) / tiles);
@@ -368,22 +417,42 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
     if (splits > cap) splits = (int)cap;
     if (splits < 1) splits = 1;
   }
-  int kchunk = K;
-  double *P = nullptr;
-  size_t mark = 0;
+  kchunk = K;
   if (splits > 1) {
     kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
     splits = (K + kchunk - 1) / kchunk;
   }
-  if (splits > 1) {
-    mark = ld_ws_mark();
-    P = (double *)ld_ws_alloc((size_t)splits * M * N * sizeof(double));
-  }
+}
+
+static void dispatch_tile(const ld_gemm_t *g, int bm, int bn, int splits, int kchunk, double *P,
+                          hipStream_t st) {
 #define DISPATCH(BM_, BN_) if (bm == BM_ && bn == BN_) launch_tile<BM_, BN_>(g, splits, kchunk, P, st)
   DISPATCH(128, 128); else DISPATCH(128, 64); else DISPATCH(128, 32);
   else DISPATCH(64, 128); else DISPATCH(64, 64); else DISPATCH(64, 32);
   else DISPATCH(32, 128); else DISPATCH(32, 64); else DISPATCH(32, 32);
 #undef DISPATCH
+}
+
+extern "C" void ld_gemm(const ld_gemm_t *g) {
+  if (g->m <= 0 || g->n <= 0) return;
+  hipStream_t st = (hipStream_t)ld_stream();
+  if (g->k <= 0 || g->alpha == 0.0) {
+    // C = beta C (rows < zero_rows -> 0)
+    ld_gemm_t h = *g;
+    h.k = 0;
+    launch_tile<64, 64>(&h, 1, 0, nullptr, st);
+    return;
+  }
+  const int M = g->m, N = g->n;
+  int bm, bn, splits, kchunk;
+  gemm_plan(g, bm, bn, splits, kchunk);
+  double *P = nullptr;
+  size_t mark = 0;
+  if (splits > 1) {
+    mark = ld_ws_mark();
+    P = (double *)ld_ws_alloc((size_t)splits * M * N * sizeof(double));
+  }
+  dispatch_tile(g, bm, bn, splits, kchunk, P, st);
   if (splits > 1) {
     size_t tot = (size_t)M * N;
     const int ph = ld_prof_begin("splitk_reduce_kernel");
@@ -393,4 +462,32 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
     ld_prof_end(ph, (double)splits * tot, 8.0 * ((double)splits * tot + 2.0 * tot));
     ld_ws_release(mark);
   }
+}
+
+// W (m x n, ldw) = op(T) (alpha op(A) op(B)) with op(T) = T^T (transT) or T,
+// T upper m x m: the split-K partials of the product go straight into one
+// launch that sums each column and multiplies it by op(T).  Returns 1 and
+// does nothing when the product would not be split, m > 64 or the GEMMs run
+// in FP32 (the caller then runs the product and the triangular multiply as
+// two GEMMs).  beta, C, zero_rows and c_upper of g are not used.
+extern "C" int ld_gemm_tmul(const ld_gemm_t *g, const double *T, int ldt, int transT, double *W,
+                            int ldw) {
+  // (m up to 128 measured no faster: two summing groups per row, 130 KB of LDS)
+  if (g->m <= 0 || g->n <= 0 || g->k <= 0 || g->alpha == 0.0 || g->m > 64 || ld_tune()->gemm_lowp)
+    return 1;
+  int bm, bn, splits, kchunk;
+  gemm_plan(g, bm, bn, splits, kchunk);
+  if (splits <= 1) return 1;
+  hipStream_t st = (hipStream_t)ld_stream();
+  const int M = g->m, N = g->n;
+  const size_t mark = ld_ws_mark();
+  double *P = (double *)ld_ws_alloc((size_t)splits * M * N * sizeof(double));
+  dispatch_tile(g, bm, bn, splits, kchunk, P, st);
+  const int ph = ld_prof_begin("splitk_reduce_tmul_kernel");
+  hipLaunchKernelGGL(splitk_reduce_tmul_kernel<64>, dim3(N), dim3(256), 0, st, M, N, splits,
+                     g->alpha, P, T, ldt, transT, W, ldw);
+  ld_prof_end(ph, (double)splits * M * N + (double)M * M * N,
+              8.0 * ((double)splits * M * N + (double)M * N));
+  ld_ws_release(mark);
+  return 0;
 }

@@ -45,6 +45,11 @@ static inline void ld_gemm_simple(int ta, int tb, int m, int n, int k, double al
     ld_gemm(&g);
 }
 
+/* W (m x n, ldw) = op(T) (alpha op(A) op(B)), op(T) = T^T (transT) or T
+ * with T upper m x m, when the product is split over K and m <= 64: the
+ * split partials and one launch that sums and multiplies; returns 1 and
+ * does nothing otherwise (beta, C, zero_rows, c_upper of g unused) */
+int ld_gemm_tmul(const ld_gemm_t *g, const double *T, int ldt, int transT, double *W, int ldw);
 /* element-wise helpers */
 void ld_lacpy(int uplo, int m, int n, const double *A, int lda, double *B, int ldb);
 void ld_laset(int uplo, int m, int n, double offdiag, double diag, double *A, int lda);
