@@ -4,9 +4,10 @@
  * one parameter at a time (coordinate descent from the current settings):
  *   - the T block height mt and the recursion cutoff nx (handed back to the
  *     caller, who owns the T layout and lila_param[3]);
- *   - the GEMM tile shape, the split-K occupancy target and the small-output
- *     tile threshold, and the slab height of the Cholesky-QR2 panel (kept in
- *     the library's tunables for the rest of the process).
+ *   - the GEMM tile shapes (of products with and without a transposed A),
+ *     the split-K occupancy target, the small-output tile thresholds and the
+ *     slab height of the Cholesky-QR2 panel (kept in the library's tunables
+ *     for the rest of the process).
  * No counterpart in the reference, whose mt, nx and block sizes are chosen
  * by hand on the command line of its test drivers.
  */
@@ -155,13 +156,17 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
         sweep(&c, "small_tiles", &tu->gemm_small_tiles, v, 3);
     }
     {
+        static const int v[] = {32, 64, 128, 256};
+        sweep(&c, "small_tiles_t", &tu->gemm_small_tiles_t, v, 4);
+    }
+    {
         static const int v[] = {96, 128, 192, 256};
         sweep(&c, "panel_rows", &tu->panel_rows_cqr, v, 4);
     }
     logf_(&c, "; chose mt %d nx %d tile %dx%d tile_t %dx%d min_wgs %d small_tiles %d "
-              "panel_rows %d: %.3f ms (%d trials)",
+              "small_tiles_t %d panel_rows %d: %.3f ms (%d trials)",
           c.mt, c.nx, tu->gemm_bm, tu->gemm_bn, tu->gemm_bm_t, tu->gemm_bn_t, tu->gemm_min_wgs,
-          tu->gemm_small_tiles, tu->panel_rows_cqr, c.best * 1e3, c.trials);
+          tu->gemm_small_tiles, tu->gemm_small_tiles_t, tu->panel_rows_cqr, c.best * 1e3, c.trials);
     *mt = c.mt;
     *nx = c.nx;
     ld_dev_free(A0); ld_dev_free(A); ld_dev_free(Q); ld_dev_free(T);

@@ -396,11 +396,12 @@ static void gemm_plan(const ld_gemm_t *g, int &bm, int &bn, int &splits, int &kc
   long tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
   // too few tiles to fill the chip: step down to 64 x 128 (from a larger
   // tile), then to 64 x 64
-  if (tiles < tu->gemm_small_tiles && bm * bn > 64 * 128 && N > 64) {
+  const int small = g->ta ? tu->gemm_small_tiles_t : tu->gemm_small_tiles;
+  if (tiles < small && bm * bn > 64 * 128 && N > 64) {
     bm = 64; bn = 128;
     tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
   }
-  if (tiles < tu->gemm_small_tiles && bm * bn > 64 * 64) {
+  if (tiles < small && bm * bn > 64 * 64) {
     bm = 64; bn = 64;
     tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
   }

@@ -171,6 +171,7 @@ typedef struct {
     int gemm_bm_t, gemm_bn_t; /* tile of products with op(A) = A^T (W = V^T C) */
     int gram_prec;           /* Cholesky-QR of the hr paths: 0 one FP64 pass; 1, 2, 3 two
                                 passes whose first Gram is FP64, FP32 MFMA, split BF16 */
+    int gemm_small_tiles_t;  /* gemm_small_tiles for products with op(A) = A^T */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

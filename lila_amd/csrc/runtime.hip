@@ -51,7 +51,7 @@ struct State {
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
-  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0};
+  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64};
 };
 State g;
 }  // namespace
@@ -101,6 +101,8 @@ int ld_init(void) {
   }
   e = getenv("LILA_GEMM_SMALL_TILES");
   if (e) g.tune.gemm_small_tiles = atoi(e);
+  e = getenv("LILA_GEMM_SMALL_TILES_T");
+  if (e) g.tune.gemm_small_tiles_t = atoi(e);
   e = getenv("LILA_PANEL_MODE");
   if (e) g.tune.panel_mode = atoi(e) ? 1 : 0;
   g.inited = 1;
