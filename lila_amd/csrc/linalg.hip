@@ -282,6 +282,47 @@ __global__ void flip_urows_where_kernel(int n, double *U, int ldu, const double 
   if (k < n && c < n && c >= k && S[k] == which) U[(size_t)k + (size_t)c * ldu] = -U[(size_t)k + (size_t)c * ldu];
 }
 
+
+// Column j's share of ||I - G||_F^2 for the symmetric G whose upper triangle
+// is stored: 2 sum_{i<j} G_ij^2 + (1 - G_jj)^2.  One workgroup of 256 per
+// column, strided partials reduced in a fixed tree, so the sum does not
+// depend on timing.
+__global__ void orth_defect_cols_kernel(int n, const double *__restrict__ G, int ldg,
+                                        double *__restrict__ part) {
+  __shared__ double red[256];
+  const int j = blockIdx.x, t = threadIdx.x;
+  const double *c = G + (size_t)j * ldg;
+  double s = 0.0;
+  for (int i = t; i < j; i += 256) s += c[i] * c[i];
+  s *= 2.0;
+  if (t == 0) {
+    const double d = 1.0 - c[j];
+    s += d * d;
+  }
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) part[j] = red[0];
+}
+
+// out[0] = sqrt(sum of part[0..n)), one workgroup, fixed order
+__global__ void sum_sqrt_kernel(int n, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int i = t; i < n; i += 256) s += part[i];
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[0] = sqrt(red[0]);
+}
+
 }  // namespace
 
 extern "C" {
@@ -423,6 +464,24 @@ void ld_getrf_b(int ml, int n, double *A, int lda, double *Q, int ldq, double *U
   hipLaunchKernelGGL(flip_urows_where_kernel, dim3((n + 255) / 256, n), dim3(256), 0, S(), n, U, n,
                      Sg, 1.0);
   ld_ws_release(mark);
+}
+
+
+// ||I - G||_F for the symmetric n x n G whose upper triangle is stored; waits
+// for the current lane's stream.  Used by the mixed-precision driver to
+// decide whether a second Cholesky-QR pass is needed.
+double ld_orth_defect(int n, const double *G, int ldg) {
+  if (n <= 0) return 0.0;
+  size_t mark = ld_ws_mark();
+  double *part = (double *)ld_ws_alloc(sizeof(double) * ((size_t)n + 1));
+  hipLaunchKernelGGL(orth_defect_cols_kernel, dim3(n), dim3(256), 0, S(), n, G, ldg, part);
+  hipLaunchKernelGGL(sum_sqrt_kernel, dim3(1), dim3(256), 0, S(), n, part, part + n);
+  double d = NAN;
+  if (hipMemcpyAsync(&d, part + n, sizeof d, hipMemcpyDeviceToHost, S()) != hipSuccess ||
+      hipStreamSynchronize(S()) != hipSuccess)
+    d = NAN;
+  ld_ws_release(mark);
+  return d;
 }
 
 }  // extern "C"

@@ -94,6 +94,9 @@ def test_mixed_matches_the_fp64_outputs(lila, ora):
 
 @pytest.mark.parametrize("kappa", [1e2, 1e5, 1e8])
 def test_mixed_accuracy_per_condition_number(lila, kappa):
+    """kappa = 1e2 leaves Q1 = A R0^-1 close enough to orthonormal
+    (||I - Q1^T Q1||_F <= 0.5) for one Cholesky-QR pass; kappa = 1e8 needs
+    the second -- both branches must reach the FP64 path's accuracy"""
     m, n, mt, nx = 2000, 800, 64, 32
     say("mixed kappa", kappa)
     rng = np.random.default_rng(int(np.log10(kappa)))
