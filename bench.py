@@ -152,6 +152,69 @@ def mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, steps, bwd_p
     return out
 
 
+def mixed_tall_leg(lila_amd, torch, dev, steps, m=200000, n=1024, mt=512, nx=32):
+    """the tall shape of the mixed-precision workload: plain and mixed forms
+    on A = U diag(sigma) V^T, m = 200000, n = 1024, kappa in {1e2, 1e5, 1e8}
+    (U, V: Q factors of seeded Gaussians from the plain path, sigma geometric
+    from 1 to 1/kappa); time per factorization (mean of `steps` after one
+    untimed run of each form), TFLOP/s by the plain VRTQ count, and the
+    errors of each form"""
+    f64 = torch.float64
+    pm = np.array([1, 1, 1, nx], dtype=np.int32)
+    pp = np.array([0, 1, 1, nx], dtype=np.int32)
+    A = torch.empty((n, m), dtype=f64, device=dev)  # column-major m x n
+    Q = torch.empty_like(A)
+    T = torch.zeros((n, mt), dtype=f64, device=dev)
+
+    def run(p, src):
+        A.copy_(src)
+        r = lila_amd.call("lila_dgeqrf_w03_recursive", p, m, n, 0, mt, A, m, T, mt, Q, m, None, 0)
+        if r != 0:
+            raise RuntimeError("lila_dgeqrf_w03_recursive (mode %d) returned %d" % (p[0], r))
+
+    G = torch.empty_like(A)
+    lila_amd.call("lila_fill_normal", ctypes.c_ulonglong(303), m, n, G, m)
+    run(pp, G)
+    U = Q.clone()
+    Gn = torch.empty((n, n), dtype=f64, device=dev)
+    lila_amd.call("lila_fill_normal", ctypes.c_ulonglong(404), n, n, Gn, n)
+    An = Gn.clone()
+    Qn = torch.empty_like(Gn)
+    Tn = torch.zeros((n, mt), dtype=f64, device=dev)
+    r = lila_amd.call("lila_dgeqrf_w03_recursive", pp, n, n, 0, mt, An, n, Tn, mt, Qn, n, None, 0)
+    if r != 0:
+        raise RuntimeError("lila_dgeqrf_w03_recursive returned %d" % r)
+    V = Qn
+    del G, Gn, An, Tn
+    idx = torch.arange(n, dtype=f64, device=dev)
+    Ak = torch.empty_like(A)
+    out = {"m": m, "n": n, "mt": mt, "nx": nx, "flops_per_factorization": vrtq_flops(m, n),
+           "by_kappa": []}
+    for kappa in (1e2, 1e5, 1e8):
+        sig = kappa ** (-idx / (n - 1))
+        US = U * sig[:, None]
+        lila_amd.call("lila_dgemm", ctypes.c_char(b"N"), ctypes.c_char(b"T"), m, n, n, 1.0, US, m, V,
+                      n, 0.0, Ak, m)
+        del US
+        row = {"kappa": kappa}
+        for name, p in (("plain", pp), ("mixed", pm)):
+            run(p, Ak)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                run(p, Ak)
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / steps
+            b, o = qr_errors(lila_amd, torch, dev, Ak, A, Q, m, n)
+            row[name] = {"ms": t * 1e3, "tflops": vrtq_flops(m, n) / t / 1e12,
+                         "backward_error": b, "orthogonality": o}
+        row["mixed_speedup_vs_plain"] = row["plain"]["ms"] / row["mixed"]["ms"]
+        out["by_kappa"].append(row)
+    del A, Q, T, U, V, Ak
+    torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summaries (profiles/*_traffic.md: FETCH_SIZE and WRITE_SIZE runs scaled
@@ -533,7 +596,8 @@ def main():
     ap.add_argument("--no-ls", action="store_true",
                     help="skip the large least-squares leg (m=100000, n=1024, 64 right-hand sides)")
     ap.add_argument("--no-mixed", action="store_true",
-                    help="skip the mixed-precision leg and its per-kappa accuracy table")
+                    help="skip the mixed-precision legs (m=n=16384 and m=200000, n=1024) "
+                         "and their per-kappa accuracy tables")
     ap.add_argument("--autotune", action="store_true",
                     help="run lila_autotune on the workload shape first (from --mt/--nx) and "
                          "measure with the settings it picks")
@@ -631,8 +695,10 @@ def main():
     del G
 
     mixed = None
+    mixed_tall = None
     if not args.no_mixed:
         mixed = mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, args.steps, bwd, orth, dt)
+        mixed_tall = mixed_tall_leg(lila_amd, torch, dev, args.steps)
 
     ls = None
     if not args.no_ls:
@@ -691,6 +757,7 @@ def main():
                    "flops_per_step": flops},
         "accuracy": {"backward_error": bwd, "orthogonality": orth, "device_status": info},
         "mixed": mixed,
+        "mixed_tall": mixed_tall,
         "batch": batch,
         "tall_cholqr": tall,
         "least_squares": ls,

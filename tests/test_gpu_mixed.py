@@ -111,3 +111,43 @@ def test_mixed_accuracy_per_condition_number(lila, kappa):
     say("  plain bwd %.2e orth %.2e | mixed bwd %.2e orth %.2e" % (bp, op, bm, om))
     assert bm < 1e-13 and om < 1e-13
     assert bm < 20 * bp + 1e-15 and om < 20 * op + 1e-14
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e5, 1e8])
+def test_mixed_tall_200000_by_1024(lila, kappa):
+    """the tall shape of the mixed-precision workload, m = 200000, n = 1024,
+    A = U diag(sigma) V^T (U, V from torch's FP64 QR of seeded Gaussians):
+    backward error of both forms within 1e-13; ||I - Q^T Q||_F (which grows
+    with n: the plain form gives ~3 n u here) within 30 n u, LAPACK's test
+    threshold on such ratios; the mixed form's errors within 20x the plain
+    form's"""
+    m, n, mt, nx = 200_000, 1024, 512, 32
+    say("mixed tall", m, n, kappa)
+    f64 = torch.float64
+    g = torch.Generator(device="cuda").manual_seed(int(np.log10(kappa)))
+    U, _ = torch.linalg.qr(torch.randn(m, n, dtype=f64, device="cuda", generator=g))
+    V, _ = torch.linalg.qr(torch.randn(n, n, dtype=f64, device="cuda", generator=g))
+    sig = kappa ** (-torch.arange(n, dtype=f64, device="cuda") / (n - 1))
+    A0 = ((U * sig) @ V.T.contiguous()).T.contiguous()  # column-major storage: (n, m)
+    del U, V
+    nA = torch.linalg.norm(A0).item()
+    res = {}
+    for mode in (0, 1):
+        A = A0.clone()
+        Q = torch.zeros_like(A0)
+        T = torch.zeros((n, mt), dtype=f64, device="cuda")
+        p = np.array([mode, 1, 1, nx], dtype=np.int32)
+        assert lila.call("lila_dgeqrf_w03_recursive", p, m, n, 0, mt, A, m, T, mt, Q, m, None, 0) == 0
+        torch.cuda.synchronize()
+        assert lila.call("lila_get_info") == 0
+        Qm = Q.T  # (m, n) view
+        R = torch.triu(A.T[:n])
+        bwd = torch.linalg.norm(A0.T - Qm @ R).item() / nA
+        orth = torch.linalg.norm(Qm.T @ Qm - torch.eye(n, dtype=f64, device="cuda")).item()
+        res[mode] = (bwd, orth)
+        del A, Q, T, R
+    (bp, op), (bm, om) = res[0], res[1]
+    say("  plain bwd %.2e orth %.2e | mixed bwd %.2e orth %.2e" % (bp, op, bm, om))
+    assert bp < 1e-13 and op < 30 * n * U64
+    assert bm < 1e-13 and om < 30 * n * U64
+    assert bm < 20 * bp + 1e-15 and om < 20 * op + 1e-14
