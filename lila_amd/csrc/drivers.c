@@ -240,11 +240,29 @@ int dv_dorgh2(int m, int n, int i, int j, int mt, double *A, int lda, double *T,
     return 0;
 }
 
+/* X = L^-1 B for the unit lower triangle L of the reconstructed columns
+ * [i, i + jb) (rows i.., ld lda), B jb x nrhs (ld jb), by the blocks the
+ * reconstruction made (the first mt - i % mt columns, then mt each): per
+ * block X_k = Linv_k B_k with the block's cached inverse (Linv: mt x jb,
+ * block k's inverse in its columns), then the rows below -= L_below X_k.
+ * B is overwritten by the partial sums; X receives the solution. */
+static void unit_lower_solve_blocked(int jb, int nrhs, int i, int mt, const double *A, int lda,
+                                     const double *Linv, double *B, double *X) {
+    for (int c = 0, w = mt - i % mt; c < jb; c += w, w = mt) {
+        if (w > jb - c) w = jb - c;
+        gemm(N_, N_, w, nrhs, w, 1.0, Linv + (size_t)c * mt, mt, LD_GEN, B + c, jb, LD_GEN, 0.0, X + c,
+             jb);
+        if (c + w < jb)
+            gemm(N_, N_, jb - c - w, nrhs, w, -1.0, A + i + c + w + (size_t)(i + c) * lda, lda, LD_GEN,
+                 X + c, jb, LD_GEN, 1.0, B + c + w, jb);
+    }
+}
+
 /* blocked reconstruction step, lila-c/lila_ormhr2_w03_hr.c:3-84 (S kept
- * apart from the workspace) */
-int dv_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int lda, double *T, int ldt,
-                     double *Q, int ldq, double *S) {
-    (void)l;
+ * apart from the workspace); Linv (or NULL: a plain TRSM, :33) holds the
+ * inverses of the earlier blocks' unit lower diagonal blocks */
+static int ormhr2_step(int m, int n, int i, int j, int mt, double *A, int lda, double *T, int ldt,
+                       double *Q, int ldq, double *S, const double *Linv) {
     int jb = j - i;
     double *Tjj = T + j % mt + (size_t)j * ldt;
     double *Ajj = A + j + (size_t)j * lda;
@@ -255,7 +273,13 @@ int dv_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int l
     ld_lacpy('A', n, n, Q + j + (size_t)j * ldq, ldq, Tjj, ldt);
     ld_lacpy('A', m - j - n, n, Q + j + n + (size_t)j * ldq, ldq, Ajj + n, lda);
     if (jb > 0) {
-        ld_trsm('L', 'L', 'N', 'U', jb, n, 1.0, A + i + (size_t)i * lda, lda, W2, jb);
+        if (Linv) {
+            double *X = wsd((size_t)jb * n);
+            unit_lower_solve_blocked(jb, n, i, mt, A, lda, Linv, W2, X);
+            W2 = X;
+        } else {
+            ld_trsm('L', 'L', 'N', 'U', jb, n, 1.0, A + i + (size_t)i * lda, lda, W2, jb);
+        }
         gemm(N_, N_, n, n, jb, -1.0, A + j + (size_t)i * lda, lda, LD_GEN, W2, jb, LD_GEN, 1.0, Tjj, ldt);
         gemm(N_, N_, m - j - n, n, jb, -1.0, A + j + n + (size_t)i * lda, lda, LD_GEN, W2, jb, LD_GEN,
              1.0, Ajj + n, lda);
@@ -265,17 +289,37 @@ int dv_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int l
     return 0;
 }
 
+/* lila-c/lila_ormhr2_w03_hr.c:3-84 (the API entry: no cached inverses) */
+int dv_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int lda, double *T, int ldt,
+                     double *Q, int ldq, double *S) {
+    (void)l;
+    return ormhr2_step(m, n, i, j, mt, A, lda, T, ldt, Q, ldq, S, NULL);
+}
+
 /* blocked reconstruction driver (loop of lila-c/lila_dgeqrf_w03_mt_hr.c:23-29);
- * Sd: device doubles indexed by global column */
+ * Sd: device doubles indexed by global column.  Each step solves with the
+ * unit lower triangle of all earlier blocks' V; their diagonal blocks do not
+ * change after their own step, so each is inverted once (a w x w TRSM on
+ * the identity) and the later solves run by blocks of mt rows, two GEMMs
+ * per block, instead of a TRSM recursion down to 64-row leaves per step. */
 int dv_dorghr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,
               double *Sd) {
+    size_t mark = ld_ws_mark();
+    double *Linv = n > mt - i % mt ? wsd((size_t)mt * n) : NULL;
     int j = i, vb = mt - i % mt;
     if (vb > n) vb = n;
     while (vb > 0) {
-        dv_ormhr2_w03_hr(m, vb, i, j, -1, mt, A, lda, T, ldt, Q, ldq, Sd);
+        ormhr2_step(m, vb, i, j, mt, A, lda, T, ldt, Q, ldq, Sd, Linv);
+        const int next = (j + vb + mt >= i + n) ? (i + n - j - vb) : mt;
+        if (Linv && next > 0) {
+            double *Lk = Linv + (size_t)(j - i) * mt;
+            ld_laset('A', vb, vb, 0.0, 1.0, Lk, mt);
+            ld_trsm('L', 'L', 'N', 'U', vb, vb, 1.0, A + j + (size_t)j * lda, lda, Lk, mt);
+        }
         j += vb;
-        vb = (j + mt >= i + n) ? (i + n - j) : mt;
+        vb = next;
     }
+    ld_ws_release(mark);
     return 0;
 }
 
@@ -432,7 +476,9 @@ int dv_dgeqrf_mixed(const int *p, int m, int n, int i, int mt, double *A, int ld
     double *G = wsd((size_t)n * n), *R = wsd((size_t)n * n), *Sd = wsd((size_t)(i + n));
     /* R0: FP32-MFMA updates, FP64 panels */
     ld_lacpy('A', ml, n, Aii, lda, W, ml);
-    int p0[4] = {0, 1, 1, p[3] > 1 ? p[3] : 32};
+    /* leaves of at most nx columns, as in the FP64 recursion (the VRT
+     * recursion stops at n < lila_param[3], the VRTQ one at n <= it) */
+    int p0[4] = {0, 1, 1, (p[3] > 1 ? p[3] : 32) + 1};
     ld_tune_t *tu = ld_tune();
     const int old = tu->gemm_lowp;
     tu->gemm_lowp = 1;
