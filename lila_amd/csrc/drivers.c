@@ -473,7 +473,8 @@ int dv_dgeqrf_mixed(const int *p, int m, int n, int i, int mt, double *A, int ld
     double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
     size_t mark = ld_ws_mark();
     double *W = wsd((size_t)ml * n), *T0 = wsd((size_t)mt * n);
-    double *G = wsd((size_t)n * n), *R = wsd((size_t)n * n), *Sd = wsd((size_t)(i + n));
+    double *G = wsd((size_t)n * n), *R = wsd((size_t)n * n), *R2 = wsd((size_t)n * n);
+    double *Sd = wsd((size_t)(i + n));
     /* R0: FP32-MFMA updates, FP64 panels */
     ld_lacpy('A', ml, n, Aii, lda, W, ml);
     /* leaves of at most nx columns, as in the FP64 recursion (the VRT
@@ -502,7 +503,12 @@ int dv_dgeqrf_mixed(const int *p, int m, int n, int i, int mt, double *A, int ld
         if (pass == 0 && ld_orth_defect(n, G, n) <= 0.5) npass = 1;
         ld_potrf_u(n, G, n);
         ld_trsm('R', 'U', 'N', 'N', ml, n, 1.0, G, n, Qii, ldq);
-        ld_trmm('L', 'U', 'N', 'N', n, n, 1.0, G, n, R, n);
+        /* R <- Rc R: both upper triangular, so the GEMM's k range per tile
+         * covers only their overlap */
+        gemm(N_, N_, n, n, n, 1.0, G, n, LD_UPPER, R, n, LD_UPPER, 0.0, R2, n);
+        double *t = R;
+        R = R2;
+        R2 = t;
     }
     /* Householder reconstruction of (Q, R) in the mt layout */
     ld_lacpy('U', n, n, R, n, Aii, lda);

@@ -186,14 +186,30 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(
     }
   };
 
+  // the k range this tile needs: where a structured op(A) (rows [m0,
+  // m0 + BM)) or op(B) (columns [n0, n0 + BN)) is zero by its triangle, the
+  // products are exact zeros, so those k steps are skipped (a product of two
+  // triangles, R = R2 R1, costs a third of a full one).  The steps kept stay
+  // on the kbeg grid; edge steps still mask element by element.
+  int klo = kbeg, khi = kend;
+  {
+    const bool a_up = (sa == LD_UPPER || sa == LD_UUNIT), a_lo = (sa == LD_LOWER || sa == LD_VUNIT);
+    const bool b_up = (sb == LD_UPPER || sb == LD_UUNIT), b_lo = (sb == LD_LOWER || sb == LD_VUNIT);
+    if (TA ? a_lo : a_up) klo = max(klo, m0);       // op(A)(i, k) = 0 for k < i
+    if (TA ? a_up : a_lo) khi = min(khi, m0 + BM);  // op(A)(i, k) = 0 for k > i
+    if (TB ? b_lo : b_up) khi = min(khi, n0 + BN);  // op(B)(k, j) = 0 for k > j
+    if (TB ? b_up : b_lo) klo = max(klo, n0);       // op(B)(k, j) = 0 for k < j
+    klo = kbeg + ((klo - kbeg) / BK) * BK;
+  }
+
   int buf = 0;
-  if (kbeg < kend) {
-    load_tiles(kbeg);
+  if (klo < khi) {
+    load_tiles(klo);
     store_tiles(0);
   }
   __syncthreads();
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    const bool more = (k0 + BK < kend);
+  for (int k0 = klo; k0 < khi; k0 += BK) {
+    const bool more = (k0 + BK < khi);
     if (more) load_tiles(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK / 4; kk++) {
