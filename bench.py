@@ -115,8 +115,8 @@ def mixed_leg(lila_amd, lib, torch, dev, A0, A, Q, T, m, n, mt, nx, steps, bwd_p
            "accuracy": {"backward_error": bwd, "orthogonality": orth},
            "plain_accuracy": {"backward_error": bwd_plain, "orthogonality": orth_plain},
            "method": "FP32-MFMA Householder R0 (FP64 panels), FP64 Cholesky-QR on A R0^-1 (a second "
-                     "pass when ||I - G||_F > 0.5 for its Gram G), Householder reconstruction "
-                     "(lila_dorghr); flops counted as plain VRTQ",
+                     "pass unless its Gram G shows ||I - G||_2 <= 0.5), Householder "
+                     "reconstruction (lila_dorghr); flops counted as plain VRTQ",
            "by_kappa": []}
     if m != n:
         return out

@@ -71,8 +71,8 @@ int lila_dlarft_w03(int m, int n, int i, int mt, double *A, int lda, double *T, 
 /* lila-c/lila.h:46-56.  lila_param[0] (mode, unused by the reference's
  * drivers) selects, in lila_dgeqrf_w03_recursive, 0: the FP64 recursion;
  * 1: mixed precision -- FP32-MFMA Householder R0 as a preconditioner, FP64
- * Cholesky-QR on A R0^-1 (a second pass when the first Gram G has
- * ||I - G||_F > 0.5), Householder reconstruction to the same V, R, T, Q
+ * Cholesky-QR on A R0^-1 (a second pass unless the first Gram G shows
+ * ||I - G||_2 <= 0.5), Householder reconstruction to the same V, R, T, Q
  * outputs (lila_param[3] is its recursion cutoff). */
 int lila_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, int m, int n, int i,
                            int mt, double *A, int lda, double *T, int ldt, double *Q, int ldq,

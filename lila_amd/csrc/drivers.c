@@ -10,6 +10,7 @@
  * steps become TRSM / SYRK / Cholesky / sign-LU launches.  All pointers are
  * device pointers; all work goes onto the library stream.
  */
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -492,15 +493,24 @@ int dv_dgeqrf_mixed(const int *p, int m, int n, int i, int mt, double *A, int ld
     ld_lacpy('U', n, n, W, ml, R, n);
     /* Cholesky-QR: G = Q^T Q = Rc^T Rc, Q <- Q Rc^-1, R <- Rc R.  A second
      * pass only when the first Gram shows Q1 away from orthonormal:
-     * ||I - G||_2 <= ||I - G||_F <= 0.5 bounds kappa(Q1)^2 by 3, and one
-     * pass then leaves ||I - Q^T Q||_F at ~0.4 n u, the Householder path's
-     * level (m = n = 16384 with a FP32-updated R0: uniform entries give
-     * ||I - G||_F = 0.16, kappa(A) = 1e4 gives 0.14, kappa(A) = 1e5 gives 1.1
-     * and takes the second pass, as does kappa(A) = 1e8 at 4.6e6). */
+     * ||I - G||_2 <= 0.5 bounds kappa(Q1)^2 by 3, and one pass then leaves
+     * ||I - Q^T Q||_F at ~0.4 n u, the Householder path's level.  First the
+     * exact bound ||I - G||_2 <= ||I - G||_F <= 0.5; failing that, unless
+     * ||I - G||_F / sqrt(n) (<= ||I - G||_2) already exceeds 1/6, a power-
+     * method estimate, which is a lower bound and so must stay below 1/6
+     * (a factor 3 margin).  At m = n = 16384 with a FP32-updated R0:
+     * uniform entries give ||I - G||_F = 0.16; kappa(A) = 1e5 gives 1.1 and
+     * ||I - G||_2 = 0.042 (one pass); kappa(A) = 1e6 gives 0.44 and 1e8
+     * gives 3.2e6 (two passes). */
     int npass = 2;
     for (int pass = 0; pass < npass; pass++) {
         ld_syrk_ut(n, ml, 1.0, Qii, ldq, 0.0, G, n);
-        if (pass == 0 && ld_orth_defect(n, G, n) <= 0.5) npass = 1;
+        if (pass == 0) {
+            const double df = ld_orth_defect(n, G, n);
+            if (df <= 0.5 ||
+                (df <= sqrt((double)n) / 6.0 && 3.0 * ld_sym_defect_norm2_est(n, G, n, 20) <= 0.5))
+                npass = 1;
+        }
         ld_potrf_u(n, G, n);
         ld_trsm('R', 'U', 'N', 'N', ml, n, 1.0, G, n, Qii, ldq);
         /* R <- Rc R: both upper triangular, so the GEMM's k range per tile

@@ -98,6 +98,7 @@ void ld_syrk_ut(int n, int k, double alpha, const double *A, int lda,
 /* upper Cholesky in place; returns through the info slot (ld_info) */
 void ld_potrf_u(int n, double *A, int lda);
 double ld_orth_defect(int n, const double *G, int ldg); /* ||I - G||_F, syncs */
+double ld_sym_defect_norm2_est(int n, const double *G, int ldg, int iters); /* <= ||I - G||_2, syncs */
 
 /* sign-chosen unpivoted LU of the n x n block Tq in place (Householder
  * reconstruction): S[k] = -1 if |1 - t_kk| < |1 + t_kk| (column negated),

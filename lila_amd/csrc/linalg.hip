@@ -308,6 +308,48 @@ __global__ void orth_defect_cols_kernel(int n, const double *__restrict__ G, int
   if (t == 0) part[j] = red[0];
 }
 
+// F = I - G in full, for the symmetric G whose upper triangle is stored
+__global__ void sym_defect_full_kernel(int n, const double *__restrict__ G, int ldg,
+                                       double *__restrict__ F) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  const double g = (i <= j) ? G[i + (size_t)j * ldg] : G[j + (size_t)i * ldg];
+  F[i + (size_t)j * n] = (i == j ? 1.0 : 0.0) - g;
+}
+
+// a fixed start vector in [-1, 1] (splitmix64 of the index)
+__global__ void start_vector_kernel(int n, double *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  x[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// part[b] = sum of y[i]^2 over the b-th 256-element slice
+__global__ void sumsq_slices_kernel(int n, const double *__restrict__ y, double *__restrict__ part) {
+  __shared__ double red[256];
+  const int t = threadIdx.x, i = blockIdx.x * 256 + t;
+  const double v = i < n ? y[i] : 0.0;
+  red[t] = v * v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = red[0];
+}
+
+// x = y / s[0] (s[0] > 0; zero vector when s[0] == 0)
+__global__ void scale_by_kernel(int n, const double *__restrict__ y, const double *__restrict__ s,
+                                double *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = s[0] > 0.0 ? y[i] / s[0] : 0.0;
+}
+
 // out[0] = sqrt(sum of part[0..n)), one workgroup, fixed order
 __global__ void sum_sqrt_kernel(int n, const double *__restrict__ part, double *__restrict__ out) {
   __shared__ double red[256];
@@ -478,6 +520,36 @@ double ld_orth_defect(int n, const double *G, int ldg) {
   hipLaunchKernelGGL(sum_sqrt_kernel, dim3(1), dim3(256), 0, S(), n, part, part + n);
   double d = NAN;
   if (hipMemcpyAsync(&d, part + n, sizeof d, hipMemcpyDeviceToHost, S()) != hipSuccess ||
+      hipStreamSynchronize(S()) != hipSuccess)
+    d = NAN;
+  ld_ws_release(mark);
+  return d;
+}
+
+// A lower estimate of ||I - G||_2 (G symmetric, upper triangle stored):
+// `iters` steps of the power method on F = I - G from a fixed start vector,
+// the last ||F x|| with ||x|| = 1.  Every reduction runs in a fixed order, so
+// the estimate does not vary from run to run.  Waits for the stream.
+double ld_sym_defect_norm2_est(int n, const double *G, int ldg, int iters) {
+  if (n <= 0) return 0.0;
+  size_t mark = ld_ws_mark();
+  const int nb = (n + 255) / 256;
+  double *F = (double *)ld_ws_alloc(sizeof(double) * (size_t)n * n);
+  double *x = (double *)ld_ws_alloc(sizeof(double) * (size_t)n);
+  double *y = (double *)ld_ws_alloc(sizeof(double) * (size_t)n);
+  double *part = (double *)ld_ws_alloc(sizeof(double) * ((size_t)nb + 1));
+  hipLaunchKernelGGL(sym_defect_full_kernel, dim3(nb, n), dim3(256), 0, S(), n, G, ldg, F);
+  hipLaunchKernelGGL(start_vector_kernel, dim3(nb), dim3(256), 0, S(), n, y);
+  for (int k = 0; k <= iters; k++) {
+    // x = y / ||y||, then (except after the last norm) y = F x
+    hipLaunchKernelGGL(sumsq_slices_kernel, dim3(nb), dim3(256), 0, S(), n, y, part);
+    hipLaunchKernelGGL(sum_sqrt_kernel, dim3(1), dim3(256), 0, S(), nb, part, part + nb);
+    if (k == iters) break;
+    hipLaunchKernelGGL(scale_by_kernel, dim3(nb), dim3(256), 0, S(), n, y, part + nb, x);
+    gemm(0, 0, n, 1, n, 1.0, F, n, LD_GEN, x, n, LD_GEN, 0.0, y, n);
+  }
+  double d = NAN;
+  if (hipMemcpyAsync(&d, part + nb, sizeof d, hipMemcpyDeviceToHost, S()) != hipSuccess ||
       hipStreamSynchronize(S()) != hipSuccess)
     d = NAN;
   ld_ws_release(mark);
