@@ -16,9 +16,22 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include", "lila.h")
 _lib = None
 
 
+def _build_script():
+    """lila_amd/build.py, loaded from its file under a name of its own:
+    importing it as the submodule lila_amd.build would replace the build()
+    function below on the package (and `from . import build` returns that
+    function, not the script)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("_lila_amd_build_script",
+                                                  os.path.join(HERE, "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def build(verbose=False):
-    from . import build as _b
-    return _b.build(verbose=verbose)
+    """Compile liblila_hip.so in-tree."""
+    return _build_script().build(verbose=verbose)
 
 
 def load():

@@ -31,3 +31,12 @@ def test_library_has_no_cpu_path_symbols():
     # the product must not link the oracle
     out = os.popen("nm -D --defined-only %s" % lila_amd.LIB_PATH).read()
     assert "ora_" not in out
+
+
+def test_build_entry_reaches_the_build_script():
+    """lila_amd.build() (called by the build entry point) finds the compile
+    script and stays callable afterwards (no compile here)"""
+    script = lila_amd._build_script()
+    assert callable(script.build)
+    assert script.LIB == lila_amd.LIB_PATH
+    assert callable(lila_amd.build)
