@@ -335,7 +335,8 @@ def test_dorgh2_global(lila, ora, m, n, i, j, mt):
     assert rel(to_host(Ad), Ao) < tol and rel(to_host(Td), To) < tol and rel(to_host(Qd), Qo) < tol
 
 
-@pytest.mark.parametrize("m,n,i,mt", [(400, 40, 0, 16), (600, 70, 9, 32), (500, 50, 3, 64)])
+@pytest.mark.parametrize("m,n,i,mt", [(400, 40, 0, 16), (600, 70, 9, 32), (500, 50, 3, 64),
+                                      (3000, 700, 13, 128)])
 def test_dorghr_blocked(lila, ora, m, n, i, mt):
     say("dorghr", m, n, i, mt)
     Qb, R = _orthonormal(ora, m - i, n, 23)
