@@ -348,6 +348,11 @@ lila_batch_plan *lila_dgels_batched_plan(int nprob, const int *m, const int *n, 
                                          double *const *A, const int *lda, double *const *B,
                                          const int *ldb);
 int lila_dgels_batched_run(lila_batch_plan *plan, int use_graph);
+/* Caps the workgroups of the persistent launch for plans made from now on
+ * (0: one per CU at the kernel's occupancy, the default); returns the
+ * previous cap.  A workgroup takes problems from the queue until it is
+ * empty, so any cap >= 1 gives the same results. */
+int lila_set_batched_grid(int wgs);
 double lila_dgels_batched_flops(const lila_batch_plan *plan);
 void lila_dgels_batched_destroy(lila_batch_plan *plan);
 

@@ -704,7 +704,12 @@ lila_batch_plan *lila_dgels_batched_plan(int nprob, const int *m, const int *n, 
     }
     for (int p = 0; p < nprob; p++) {
         if (!ld_is_device_ptr(A[p]) || (nrhs[p] > 0 && !ld_is_device_ptr(B[p]))) {
-            fprintf(stderr, "lila_hip: batched plan needs device pointers (problem %d)\n", p);
+            int e = 0, t = 0;
+            ld_ptr_diag(ld_is_device_ptr(A[p]) ? (const void *)B[p] : (const void *)A[p], &e, &t);
+            fprintf(stderr,
+                    "lila_hip: batched plan needs device pointers (problem %d: %p / %p, "
+                    "hipPointerGetAttributes status %d, memory type %d)\n",
+                    p, (void *)A[p], (void *)B[p], e, t);
             return NULL;
         }
     }

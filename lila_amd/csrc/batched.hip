@@ -78,7 +78,7 @@ struct Smem {
   double D0[PNB], D2[PNB], D3[PNB], Sv[PNB], tau[PNB], wv[PNB];
   double red[NW][PNB + 2];
   double sc[4];
-  int job, flag;
+  int job, flag, colbad;
 };
 
 // the panel being factored: first column j0, width jb, na A columns right
@@ -287,37 +287,65 @@ __device__ __noinline__ void panel_columns(double *A, int m, int lda, int j0, in
     __syncthreads();
     if (tid == 0) {
       double sst = 0.0, mxt = 0.0;
+      int bad = 0;
       for (int w = 0; w < NW; w++) {
         sst += s.red[w][0];
         mxt = fmax(mxt, s.red[w][1]);
       }
+      for (int q = 0; q < nq; q++) {
+        double dq = 0.0;
+        for (int w = 0; w < NW; w++) dq += s.red[w][2 + q];
+        s.wv[q] = dq;
+        bad |= !(fabs(dq) <= 1.7976931348623157e308);
+      }
       s.sc[3] = mxt;
       s.sc[2] = sst;
+      s.colbad = bad;
     }
     __syncthreads();
-    double xnorm;
+    double xnorm, sc = 1.0;
     const double mxt = s.sc[3];
-    if (mxt != 0.0 && (mxt < 1e-150 || mxt > 1e150)) {
-      // sum of squares out of range: a scaled pass (the purpose of dnrm2)
-      const double inv = 1.0 / mxt;
+    if (mxt != 0.0 && (mxt < 1e-150 || mxt > 1e150 || s.colbad)) {
+      // a sum of squares or a product out of range: a pass on the column
+      // scaled by a power of two near 1 / max|x| (the purpose of dnrm2's
+      // scaling), the dot products with the later columns included
+      const double inv = ldexp(1.0, -ilogb(mxt));
+      sc = ldexp(1.0, ilogb(mxt));
       double s2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < NB - 1; q++) d[q] = 0.0;
       for (int r = col + 1 + tid; r < m; r += BT) {
         const double x = A[(size_t)r + (size_t)col * lda] * inv;
         s2 += x * x;
+#pragma unroll
+        for (int q = 0; q < NB - 1; q++)
+          if (q < nq) d[q] += x * A[(size_t)r + (size_t)(col + 1 + q) * lda];
       }
       s2 = wsum(s2);
+#pragma unroll
+      for (int q = 0; q < NB - 1; q++) d[q] = wsum(d[q]);
       __syncthreads();
-      if (lane == 0) s.red[wave][0] = s2;
+      if (lane == 0) {
+        s.red[wave][0] = s2;
+#pragma unroll
+        for (int q = 0; q < NB - 1; q++) s.red[wave][2 + q] = d[q];
+      }
       __syncthreads();
       double t = 0.0;
       for (int w = 0; w < NW; w++) t += s.red[w][0];
-      xnorm = mxt * sqrt(t);
+      xnorm = sc * sqrt(t);
+      if (tid == 0)
+        for (int q = 0; q < nq; q++) {
+          double dq = 0.0;
+          for (int w = 0; w < NW; w++) dq += s.red[w][2 + q];
+          s.wv[q] = dq;
+        }
     } else {
       xnorm = sqrt(s.sc[2]);
     }
     if (tid == 0) {
       const double alpha = A[(size_t)col + (size_t)col * lda];
-      double beta = alpha, tau = 0.0, scale = 0.0;
+      double beta = alpha, tau = 0.0, scale = 0.0, dscale = 0.0;
       if (xnorm != 0.0) {
         const double aa = fabs(alpha);
         const double w = fmax(aa, xnorm), z = fmin(aa, xnorm);
@@ -325,16 +353,14 @@ __device__ __noinline__ void panel_columns(double *A, int m, int lda, int j0, in
         beta = -copysign(nrm, alpha);
         tau = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
+        dscale = sc / (alpha - beta);  // the dots are of the column scaled by 1 / sc
       }
       s.sc[0] = beta;
       s.sc[1] = tau;
       s.sc[2] = scale;
       s.tau[c] = tau;
-      for (int q = 0; q < nq; q++) {
-        double dq = 0.0;
-        for (int w = 0; w < NW; w++) dq += s.red[w][2 + q];
-        s.wv[q] = tau * (A[(size_t)col + (size_t)(col + 1 + q) * lda] + dq * scale);
-      }
+      for (int q = 0; q < nq; q++)
+        s.wv[q] = tau * (A[(size_t)col + (size_t)(col + 1 + q) * lda] + s.wv[q] * dscale);
     }
     __syncthreads();
     const double beta = s.sc[0], tau = s.sc[1], scale = s.sc[2];
@@ -751,6 +777,8 @@ extern "C" ld_batch_plan *ld_batch_plan_create(int nprob, const int *m, const in
   hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)batched_ls_kernel, BT, 0);
   if (per < 1) per = 1;
   pl->grid = ncu * per;
+  const int cap = ld_tune()->batch_grid;
+  if (cap > 0 && pl->grid > cap) pl->grid = cap;
   if (pl->grid > nprob) pl->grid = nprob;
   ld_sync();
   return pl;

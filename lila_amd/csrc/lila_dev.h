@@ -137,6 +137,7 @@ void ld_set_lane(int lane);
 void ld_lane_fork(void);
 void ld_lane_join(void);
 int  ld_is_device_ptr(const void *p);
+void ld_ptr_diag(const void *p, int *err, int *type);
 void *ld_ws_alloc(size_t bytes);  /* bump allocation from the arena */
 size_t ld_ws_mark(void);
 void ld_ws_release(size_t mark);
@@ -174,6 +175,8 @@ typedef struct {
     int gram_prec;           /* Cholesky-QR of the hr paths: 0 one FP64 pass; 1, 2, 3 two
                                 passes whose first Gram is FP64, FP32 MFMA, split BF16 */
     int gemm_small_tiles_t;  /* gemm_small_tiles for products with op(A) = A^T */
+    int batch_grid;          /* batched least squares: workgroups of the persistent launch
+                                (0: one per CU at the kernel's occupancy) */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

@@ -53,6 +53,8 @@ constexpr int PNT = 256;
 constexpr int PNB = LD_PANEL_MAX;  // 32
 constexpr int SLOT = 128;          // one reduction vector
 constexpr int SLOT_SCALED = 126;   // slot of the scaled sum of squares
+constexpr int SLOT_SDOT = 66;      // scaled dot products (a column's sums and row take slots 0..64)
+static_assert(SLOT_SDOT > 2 * PNB && SLOT_SDOT + PNB - 1 < SLOT_SCALED, "reduction slot layout");
 constexpr int LDM = 34;            // leading dimension of 32 x 32 LDS matrices (even: 16-byte pairs)
 static_assert(PNB == small32::PNB && LDM == small32::LDM, "32 x 32 helpers");
 using small32::rdlane;
@@ -205,10 +207,13 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
   double *tauL = rowj + PNB;        // tau
   double *wk = tauL + PNB;          // w_k
   double *scal = wk + PNB;          // [0] beta [1] tau [2] scale [3] xnorm [4] maxabs
+                                    // [5] scale of the dots [6] their factor in w_k
+                                    // [7] a dot is not finite
   double *lred = scal + 16;         // SOLO: local reduction vector (SLOT)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = blockIdx.x;
   double *gred = a.scr + OFF_RED;
+  if (tid == 0) scal[7] = 0.0;
 
   for (int j = 0; j <= nb; j++) {
     const int nk = (j < nb) ? nb - 1 - j : 0;
@@ -290,8 +295,12 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
     if (tid < nsum) {
       const int slot = (tid < nk) ? 2 + tid : 2 + nk + nr + (tid - nk);
       const double v = SOLO ? rv[slot] : ld_coh(&rv[slot]);
-      if (tid < nk) vals[tid] = v;
-      else vals[64 + (tid - nk)] = v;
+      if (tid < nk) {
+        vals[tid] = v;
+        if (!(fabs(v) <= 1.7976931348623157e308)) scal[7] = 1.0;
+      } else {
+        vals[64 + (tid - nk)] = v;
+      }
     }
     if (tid >= 64 && tid < 64 + nr)
       rowj[tid - 64] = SOLO ? rv[2 + nk + (tid - 64)] : ld_coh(&rv[2 + nk + (tid - 64)]);
@@ -302,22 +311,35 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
     __syncthreads();
     if (j < nb) {
       const double mx = scal[4];
-      if (mx != 0.0 && (mx < 1e-150 || mx > 1e150)) {
-        // scaled second pass: sum of (x / max)^2
-        if (SOLO && tid == 0) rv[SLOT_SCALED] = 0.0;
-        __syncthreads();
-        if (wave == 0) {
-          const double inv = 1.0 / mx;
-          double s = 0.0;
-          for (int lr = lane; lr < nloc; lr += 64) {
-            if (r0 + lr <= j) continue;
-            const double x = slab[lr + j * RS] * inv;
-            s += x * x;
+      // a product x_r a_rk out of range leaves a dot non-finite (or, with
+      // the column's entries tiny, flushes it toward zero: mx < 1e-150)
+      // (scal[7]: set while the sums were read, cleared again before the
+      // next column's; no __syncthreads_or, whose LDS word would not fit
+      // beside the dynamic LDS the launch asks for)
+      const bool anybad = scal[7] != 0.0;
+      if (mx != 0.0 && (mx < 1e-150 || mx > 1e150 || anybad)) {
+        // scaled second pass (the purpose of dnrm2's scaling): the column
+        // times a power of two near 1 / max|x|, exact; its sum of squares
+        // and its dot products with the later columns
+        const double inv = ldexp(1.0, -ilogb(mx)), sc = ldexp(1.0, ilogb(mx));
+        for (int q = wave; q < 1 + nk; q += 4) {
+          double sum = 0.0;
+          if (q == 0) {
+            for (int lr = lane; lr < nloc; lr += 64) {
+              if (r0 + lr <= j) continue;
+              const double x = slab[lr + j * RS] * inv;
+              sum += x * x;
+            }
+          } else {
+            const int k = j + q;
+            for (int lr = lane; lr < nloc; lr += 64)
+              if (r0 + lr > j) sum += (slab[lr + j * RS] * inv) * slab[lr + k * RS];
           }
-          s = wave_sum(s);
+          sum = wave_sum(sum);
           if (lane == 0) {
-            if (SOLO) rv[SLOT_SCALED] = s;
-            else if (s != 0.0) add_coh(&rv[SLOT_SCALED], s);
+            const int slot = (q == 0) ? SLOT_SCALED : SLOT_SDOT + q - 1;
+            if (SOLO) rv[slot] = sum;
+            else if (sum != 0.0) add_coh(&rv[slot], sum);
           }
         }
         if (SOLO) {
@@ -326,15 +348,20 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
           nbar++;
           grid_sync(a.cnt, nbar * (unsigned)G, a.err);
         }
-        if (tid == 0) scal[3] = mx * sqrt(SOLO ? rv[SLOT_SCALED] : ld_coh(&rv[SLOT_SCALED]));
+        if (tid == 0) {
+          scal[3] = sc * sqrt(SOLO ? rv[SLOT_SCALED] : ld_coh(&rv[SLOT_SCALED]));
+          scal[5] = sc;
+        }
+        if (tid < nk) vals[tid] = SOLO ? rv[SLOT_SDOT + tid] : ld_coh(&rv[SLOT_SDOT + tid]);
       } else if (tid == 0) {
         scal[3] = sqrt(scal[3]);
+        scal[5] = 1.0;
       }
       __syncthreads();
     }
     if (tid == 0 && j < nb) {
       const double alpha = rowj[0], xnorm = scal[3];
-      double beta = alpha, tau = 0.0, scale = 0.0;
+      double beta = alpha, tau = 0.0, scale = 0.0, dscale = 0.0;
       if (xnorm != 0.0) {
         const double aa = fabs(alpha);
         const double w = fmax(aa, xnorm), z = fmin(aa, xnorm);
@@ -342,16 +369,19 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
         beta = -copysign(nrm, alpha);
         tau = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
+        dscale = scal[5] / (alpha - beta);  // the dots are of the column divided by scal[5]
       }
       scal[0] = beta;
       scal[1] = tau;
       scal[2] = scale;
+      scal[6] = dscale;
+      scal[7] = 0.0;
       tauL[j] = tau;
     }
     __syncthreads();
     if (j < nb && tid > j && tid < nb) {
       const int k = tid;
-      wk[k] = scal[1] * (rowj[k - j] + vals[k - j - 1] * scal[2]);
+      wk[k] = scal[1] * (rowj[k - j] + vals[k - j - 1] * scal[6]);
     }
     if (j >= 1 && tid < j - 1) Gm[tid + PNB * (j - 1)] = vals[64 + tid];
     __syncthreads();
@@ -983,16 +1013,22 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   static int attr_set = 0, ncu = 0, coop = 0;
   const size_t maxb = LDS_DYN_BYTES;
   if (!attr_set) {
-    hipFuncSetAttribute((const void *)panel_hh_kernel<false>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
-    hipFuncSetAttribute((const void *)panel_hh_kernel<true>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
-    hipFuncSetAttribute((const void *)panel_cqr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)maxb);
-    hipFuncSetAttribute((const void *)panel_hh_global_kernel,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
-    hipFuncSetAttribute((const void *)panel_cqr_stream_kernel,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxb);
+    // the dynamic LDS asked for plus each kernel's static LDS (a few bytes:
+    // LDS_DYN_BYTES leaves 16) must fit the 160 KiB of a CU; a failure here
+    // is reported and cleared, not left pending for the next runtime call
+    const void *kern[5] = {(const void *)panel_hh_kernel<false>, (const void *)panel_hh_kernel<true>,
+                           (const void *)panel_cqr_kernel, (const void *)panel_hh_global_kernel,
+                           (const void *)panel_cqr_stream_kernel};
+    for (int q = 0; q < 5; q++) {
+      const hipError_t e = hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)maxb);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fprintf(stderr, "lila_hip: panel kernel %d: %zu bytes of dynamic LDS refused (%s)\n", q,
+                maxb, hipGetErrorString(e));
+        ld_set_info(-5);
+      }
+    }
     int dev = 0, per_cu = 0, per_cu2 = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1038,7 +1074,9 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
                    : glob ? sizeof(double) * LDS_TAIL : panel_smem_bytes(nb, rs);
   const int ph = ld_prof_begin(cls);
   hipError_t e;
-  if (coop && G > 1) {
+  // with two lanes the other lane's GEMMs may hold CUs while the panel
+  // launches, so co-residency is asked of the cooperative launch
+  if ((coop || ld_lanes() > 1) && G > 1) {
     e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(PNT), args, (unsigned)shb, (hipStream_t)ld_stream());
   } else {
     e = hipLaunchKernel(fn, dim3(G), dim3(PNT), args, shb, (hipStream_t)ld_stream());
@@ -1047,6 +1085,7 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   ld_prof_end(ph, 2.0 * ml * nb * nb + (Q ? 2.0 * ml * nb * nb : 0.0),
               8.0 * ((Q ? 3.0 : 2.0) * ml * nb + (double)nb * nb));
   if (e != hipSuccess) {
+    (void)hipGetLastError();
     fprintf(stderr, "lila_hip: panel launch failed (G=%d rp=%d nb=%d): %s\n", G, rp, nb,
             hipGetErrorString(e));
     ld_set_info(-1);

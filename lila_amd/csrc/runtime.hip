@@ -147,6 +147,17 @@ void ld_lane_join(void) {
   HIPCHECK(hipStreamWaitEvent(g.user_stream, e, 0));
 }
 
+// the raw classification, for error messages: hipPointerGetAttributes'
+// status and the memory type it reports
+void ld_ptr_diag(const void *p, int *err, int *type) {
+  hipPointerAttribute_t a;
+  memset(&a, 0, sizeof a);
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) (void)hipGetLastError();
+  *err = (int)e;
+  *type = (int)a.type;
+}
+
 int ld_is_device_ptr(const void *p) {
   if (!p) return 0;
   hipPointerAttribute_t a;
@@ -408,6 +419,14 @@ int lila_set_gram_precision(int prec) {
   if (prec < 0 || prec > 3) return -1;
   const int old = g.tune.gram_prec;
   g.tune.gram_prec = prec;
+  return old;
+}
+
+int lila_set_batched_grid(int wgs) {
+  if (ld_init() != 0) return -1;
+  if (wgs < 0) return -1;
+  const int old = g.tune.batch_grid;
+  g.tune.batch_grid = wgs;
   return old;
 }
 

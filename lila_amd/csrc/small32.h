@@ -70,7 +70,9 @@ __device__ __forceinline__ void chol16(double *M, double *D, int o, int nb, bool
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     const double d = rdlane(a[j], j);
-    ok = ok && (d > 0.0);
+    // a pivot must be positive and finite: an overflowed Gram (inf) or a
+    // NaN from one fails here, not only through the spread test
+    ok = ok && (d > 0.0) && (d <= 1.7976931348623157e308);
     // 1 / sqrt(d) from the hardware estimate and two Newton steps
     // y <- y (3 - d y^2) / 2 (quadratic convergence: full precision), then
     // sqrt(d) = d / sqrt(d); both within an ulp or two of the rounded values

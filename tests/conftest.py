@@ -22,6 +22,22 @@ def ora():
     return pyoracle
 
 
+@pytest.fixture(autouse=True)
+def no_pending_hip_error(request):
+    """a GPU test must leave no pending HIP error behind (one left by a
+    failed runtime call would surface at the next test's first torch call)"""
+    yield
+    if request.node.get_closest_marker("gpu") is None:
+        return
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return
+    err = hip.hipGetLastError()
+    assert err == 0, "pending HIP error %d after %s" % (err, request.node.nodeid)
+
+
 @pytest.fixture(scope="session")
 def lila():
     """the product C ABI (liblila_hip.so) with a visible GPU"""
@@ -37,11 +53,23 @@ def to_dev(A):
     column-major layout (shape (n, ld), row c = column c)"""
     import torch
     A = np.asfortranarray(A)
-    return torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    h = torch.from_numpy(np.ascontiguousarray(A.T))
+    if h.numel() * 8 > (4 << 20):
+        # large copies go through a pinned host buffer (allocated pinned, not
+        # registered after the fact)
+        p = torch.empty(h.shape, dtype=h.dtype, pin_memory=True)
+        p.copy_(h)
+        h = p
+    return h.cuda()
 
 
 def to_host(t):
     """inverse of to_dev"""
+    import torch
+    if t.numel() * 8 > (4 << 20):
+        p = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        p.copy_(t)
+        return np.asfortranarray(p.numpy().T.copy())
     return np.asfortranarray(t.cpu().numpy().T)
 
 

@@ -138,7 +138,8 @@ def test_batched_plan_graph_replays(ora, lila):
     sizes = make_sizes(31, 40, mmax=1200) + EDGE[:4]
     b = Batch(sizes, 32)
     lib = lila.load()
-    plan = lib.lila_dgels_batched_plan(*[lila.conv(a) for a in b.arrays()])
+    arrs = b.arrays()  # kept alive: the call reads the arrays behind these pointers
+    plan = lib.lila_dgels_batched_plan(*[lila.conv(a) for a in arrs])
     assert plan
     plan = ctypes.c_void_p(plan)
     try:
@@ -178,3 +179,40 @@ def test_batched_rejects_out_of_range(lila):
                       np.array([200], np.int32))
         assert r == -1, (m, n, k)
     assert lila.call("lila_dgels_batched", 0, None, None, None, None, None, None, None) == 0
+
+
+def _srel(a, b):
+    s = float(np.max(np.abs(b)))
+    return np.linalg.norm(a / s - b / s) / max(np.linalg.norm(b / s), 1e-300)
+
+
+@pytest.mark.parametrize("scale", [1e-200, 1e200])
+def test_batched_extreme_scales(ora, lila, scale):
+    """A scaled by 1e-200 or 1e200: every panel's Gram underflows to zero or
+    overflows, the panels take the column loop, whose norms and dot products
+    run on the column scaled by a power of two near 1 / max|x| (dlarfg's
+    safe scaling); V, R and X against the oracle's dgels relative to their
+    largest entries, everything finite"""
+    import torch
+    sizes = [(300, 40, 3, 300, 300), (700, 100, 5, 701, 700), (97, 33, 2, 97, 97)]
+    b = Batch(sizes, 51)
+    for p, (m, n, k, lda, ldb) in enumerate(sizes):
+        A, _ = b.views(b.host0, p)
+        A[:, :] *= scale
+    b.buf.copy_(torch.from_numpy(b.host0))
+    say("batched extreme scale", scale)
+    assert lila.call("lila_dgels_batched", *b.arrays()) == 0
+    assert lila.call("lila_sync") == 0
+    out = b.buf.cpu().numpy()
+    assert np.all(np.isfinite(out))
+    for p, (m, n, k, lda, ldb) in enumerate(sizes):
+        A0, B0 = b.views(b.host0, p)
+        Ag, Bg = b.views(out, p)
+        Ao, Bo = A0.copy(order="F"), B0.copy(order="F")
+        tau = np.zeros(n)
+        assert ora.call("dgels_hh", m, n, k, Ao, lda, Bo, ldb, tau) == 0
+        tol = 100 * n * U64
+        assert _srel(np.tril(Ag[:m], -1), np.tril(Ao[:m], -1)) < tol, ("V", p)
+        assert _srel(np.triu(Ag[:n]), np.triu(Ao[:n])) < tol, ("R", p)
+        cond = np.linalg.cond(A0[:m] / scale)
+        assert _srel(Bg[:n, :k], Bo[:n, :k]) < tol * max(1.0, cond), ("X", p)
