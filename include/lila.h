@@ -348,6 +348,18 @@ lila_batch_plan *lila_dgels_batched_plan(int nprob, const int *m, const int *n, 
                                          double *const *A, const int *lda, double *const *B,
                                          const int *ldb);
 int lila_dgels_batched_run(lila_batch_plan *plan, int use_graph);
+/* No counterpart in the reference: lila_dgels_batched for callers that
+ * cannot allocate (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The problem table, queue order and
+ * queue counter live in a static device array of the library
+ * (16384 problems at most, ~0.7 MB per GPU, held until the process ends)
+ * instead of memory allocated per call; A[p] and B[p] must be device
+ * pointers.  Enqueued on the library stream and returns (0; -1 for more
+ * than 16384 problems, sizes out of range or host pointers, with nothing
+ * launched).  Calls share the array of their GPU: two calls that overlap
+ * on one GPU (different streams, or threads) race. */
+int lila_dgels_batched_noalloc(int nprob, const int *m, const int *n, const int *nrhs,
+                               double *const *A, const int *lda, double *const *B, const int *ldb);
 /* Caps the workgroups of the persistent launch for plans made from now on
  * (0: one per CU at the kernel's occupancy, the default); returns the
  * previous cap.  A workgroup takes problems from the queue until it is

@@ -763,6 +763,28 @@ done:
     return r;
 }
 
+/* no allocation: the problem table, queue order and counter in the
+ * library's static device scratch (batched.hip); device pointers only;
+ * asynchronous on the library stream */
+int lila_dgels_batched_noalloc(int nprob, const int *m, const int *n, const int *nrhs,
+                               double *const *A, const int *lda, double *const *B, const int *ldb) {
+    if (nprob == 0) return 0;
+    if (nprob < 0 || !m || !n || !nrhs || !A || !lda || !B || !ldb) return -1;
+    if ((size_t)nprob > ld_batch_noalloc_max()) {
+        fprintf(stderr, "lila_hip: lila_dgels_batched_noalloc takes at most %zu problems (got %d)\n",
+                ld_batch_noalloc_max(), nprob);
+        return -1;
+    }
+    for (int p = 0; p < nprob; p++) {
+        if (!ld_is_device_ptr(A[p]) || (nrhs[p] > 0 && !ld_is_device_ptr(B[p]))) {
+            fprintf(stderr, "lila_hip: lila_dgels_batched_noalloc needs device pointers (problem %d)\n",
+                    p);
+            return -1;
+        }
+    }
+    return ld_batch_noalloc_run(nprob, m, n, nrhs, A, lda, B, ldb);
+}
+
 void lila_set_stream(void *s) { ld_set_stream(s); }
 void *lila_get_stream(void) { return ld_stream(); }
 

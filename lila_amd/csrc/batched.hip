@@ -817,6 +817,113 @@ extern "C" int ld_batch_plan_run(ld_batch_plan *pl, int use_graph) {
 
 extern "C" double ld_batch_plan_flops(const ld_batch_plan *pl) { return pl ? pl->flops : 0.0; }
 
+// ---------------------------------------------------------------------------
+// Batched least squares without allocation: the problem table, the queue
+// order and the queue counter live in `scratch`, a file-scope device array
+// (one copy per GPU, held until the process ends).  Host code never uses the
+// array's name as an address: its device address is looked up once per
+// device with hipGetSymbolAddress and kept in na_ptr[device].  Every call on
+// a GPU shares that GPU's array; calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int NA_MAXPROB = 16384;  // problems per call at most
+constexpr size_t NA_TAB = sizeof(BProb) * NA_MAXPROB;
+constexpr size_t NA_ORD = sizeof(int) * NA_MAXPROB;
+constexpr size_t NA_BYTES = NA_TAB + NA_ORD + 256;  // + the queue counter
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+constexpr int NA_MAXDEV = 64;
+void *na_ptr[NA_MAXDEV];
+int na_grid[NA_MAXDEV];
+// host staging of the table (a pageable source: hipMemcpyAsync copies it
+// synchronously, so it may be refilled by the next call)
+BProb na_tab[NA_MAXPROB];
+int na_ord[NA_MAXPROB];
+std::pair<double, int> na_key[NA_MAXPROB];
+}  // namespace
+
+// once per device, with that device current: the array's device address
+// and the persistent grid
+static int na_setup(int dev) {
+  if (na_ptr[dev]) return 0;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    fprintf(stderr, "lila_hip: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+    return -1;
+  }
+  int ncu = 0, per = 0;
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)batched_ls_kernel, BT, 0);
+  na_grid[dev] = (ncu > 0 ? ncu : 1) * (per > 0 ? per : 1);
+  na_ptr[dev] = ptr;
+  return 0;
+}
+
+extern "C" size_t ld_batch_noalloc_max(void) { return NA_MAXPROB; }
+
+// set-up for the current device (the library's init call runs it)
+extern "C" int ld_batch_noalloc_setup(void) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAXDEV) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  return na_setup(dev);
+}
+
+// enqueue the batch on the library stream; 0, or -1 for arguments out of
+// range (nothing launched)
+extern "C" int ld_batch_noalloc_run(int nprob, const int *m, const int *n, const int *k,
+                                    double *const *A, const int *lda, double *const *B,
+                                    const int *ldb) {
+  if (nprob == 0) return 0;
+  if (nprob < 0 || nprob > NA_MAXPROB) {
+    fprintf(stderr, "lila_hip: lila_dgels_batched_noalloc takes at most %d problems (got %d)\n",
+            NA_MAXPROB, nprob);
+    return -1;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAXDEV) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  if (na_setup(dev) != 0) return -1;
+  for (int i = 0; i < nprob; i++) {
+    if (m[i] < n[i] || n[i] < 1 || n[i] > MAXN || k[i] < 0 || k[i] > MAXK || lda[i] < m[i] ||
+        (k[i] > 0 && ldb[i] < m[i])) {
+      fprintf(stderr, "lila_hip: batched problem %d out of range (m=%d n=%d k=%d)\n", i, m[i], n[i],
+              k[i]);
+      return -1;
+    }
+    na_tab[i] = BProb{m[i], n[i], k[i], lda[i], ldb[i], 0, A[i], B[i]};
+    // queue order: largest flop count first, ties by index (the plan's order)
+    na_key[i] = std::make_pair(-prob_flops(m[i], n[i], k[i]), i);
+  }
+  std::sort(na_key, na_key + nprob);  // in place, no allocation
+  for (int i = 0; i < nprob; i++) na_ord[i] = na_key[i].second;
+  hipStream_t st = (hipStream_t)ld_stream();
+  char *base = (char *)na_ptr[dev];
+  BProb *dprob = (BProb *)base;
+  int *dorder = (int *)(base + NA_TAB);
+  int *dcounter = (int *)(base + NA_TAB + NA_ORD);
+  if (hipMemcpyAsync(dprob, na_tab, sizeof(BProb) * nprob, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dorder, na_ord, sizeof(int) * nprob, hipMemcpyHostToDevice, st) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  const int grid = na_grid[dev] < nprob ? na_grid[dev] : nprob;
+  // the counter is reset by a kernel, not a memset (the last call's value stays)
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, st, dcounter);
+  hipLaunchKernelGGL(batched_ls_kernel, dim3(grid), dim3(BT), 0, st, dprob, dorder, nprob, dcounter);
+  if (hipGetLastError() != hipSuccess) return -1;
+  return 0;
+}
+
 extern "C" void ld_batch_plan_destroy(ld_batch_plan *pl) {
   if (!pl) return;
   ld_sync();

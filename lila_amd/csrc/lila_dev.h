@@ -189,6 +189,12 @@ ld_batch_plan *ld_batch_plan_create(int nprob, const int *m, const int *n, const
 int ld_batch_plan_run(ld_batch_plan *pl, int use_graph);
 double ld_batch_plan_flops(const ld_batch_plan *pl);
 void ld_batch_plan_destroy(ld_batch_plan *pl);
+/* the same launch with the problem table, queue order and counter in a
+ * static device array instead of allocated memory (lila_dgels_batched_noalloc) */
+int ld_batch_noalloc_setup(void);
+size_t ld_batch_noalloc_max(void);
+int ld_batch_noalloc_run(int nprob, const int *m, const int *n, const int *k, double *const *A,
+                         const int *lda, double *const *B, const int *ldb);
 
 #ifdef __cplusplus
 }

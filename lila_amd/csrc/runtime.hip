@@ -105,6 +105,8 @@ int ld_init(void) {
   if (e) g.tune.gemm_small_tiles_t = atoi(e);
   e = getenv("LILA_PANEL_MODE");
   if (e) g.tune.panel_mode = atoi(e) ? 1 : 0;
+  // the static scratch of lila_dgels_batched_noalloc on the current device
+  if (ld_batch_noalloc_setup() != 0) return -1;
   g.inited = 1;
   return 0;
 }
