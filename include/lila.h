@@ -20,7 +20,12 @@
  *
  * Return value: 0 on success, a positive LAPACK-style info on numerical
  * breakdown (Cholesky not positive definite: the 1-based column), negative
- * on a device or argument error.  The reference always returns 0.
+ * on a device or argument error: -1 no device / bad arguments, -2 device
+ * memory for staging or workspace could not be allocated (nothing was
+ * computed), -4 a panel's grid barrier timed out, -6 the library stream is
+ * being captured into a hipGraph (the dense entry points refuse capture;
+ * the batched plan replays a graph of its own).  The reference always
+ * returns 0.
  * Not thread-safe: one calling thread per process.
  */
 #ifndef LILA_H_MI355X
@@ -314,7 +319,9 @@ long int flops_dlarft3(int m, int n);
 int lila_get_info(void);
 /* Wait for the library stream; returns the status like lila_get_info. */
 int lila_sync(void);
-/* Library stream (a hipStream_t); NULL selects the default stream. */
+/* Library stream (a hipStream_t); NULL selects the default stream.  The
+ * switch is ordered: work enqueued afterwards waits for the library's work
+ * already enqueued on the previous stream. */
 void lila_set_stream(void *hip_stream);
 void *lila_get_stream(void);
 

@@ -24,14 +24,24 @@ typedef struct {
 } stage_t;
 
 static int g_host_staged = 0;
+/* a staging allocation failed in this call: the driver is skipped (no
+ * kernel ever sees a NULL device pointer), the buffers already staged are
+ * freed without copying back, and the call returns -2 */
+static int g_stage_fail = 0;
 
 static stage_t stage_in(double *p, int ld, int rows, int cols) {
     stage_t s = {p, p, 0, 0};
     if (!p || rows <= 0 || cols <= 0) return s;
+    if (g_stage_fail) { s.dev = NULL; return s; }
     if (ld_is_device_ptr(p)) return s;
     s.count = (size_t)(cols - 1) * (size_t)ld + (size_t)rows;
     s.dev = (double *)ld_dev_malloc(s.count * sizeof(double));
-    if (!s.dev) { s.dev = NULL; return s; }
+    if (!s.dev) {
+        fprintf(stderr, "lila_hip: cannot stage %zu bytes of a host operand on the device\n",
+                s.count * sizeof(double));
+        g_stage_fail = 1;
+        return s;
+    }
     ld_memcpy_h2d(s.dev, p, s.count * sizeof(double));
     s.staged = 1;
     g_host_staged = 1;
@@ -40,7 +50,7 @@ static stage_t stage_in(double *p, int ld, int rows, int cols) {
 
 static void stage_out(stage_t *s, int copy_back) {
     if (!s->staged) return;
-    if (copy_back) ld_memcpy_d2h(s->user, s->dev, s->count * sizeof(double));
+    if (copy_back && !g_stage_fail) ld_memcpy_d2h(s->user, s->dev, s->count * sizeof(double));
     ld_dev_free(s->dev);
     s->staged = 0;
 }
@@ -50,13 +60,30 @@ static int api_begin(void) {
         fprintf(stderr, "lila_hip: no usable HIP device; this library has no CPU path\n");
         return -1;
     }
+    /* The drivers grow their workspace arena during a call and fold its
+     * chunks into one (freeing the old ones) when the call ends, read device
+     * status back mid-call and clear flags with memsets: a captured graph
+     * would replay into freed memory.  So they refuse capture; the batched
+     * plan (lila_dgels_batched_run) is the path to replay. */
+    if (ld_stream_capturing()) {
+        fprintf(stderr, "lila_hip: this entry point cannot run under hipGraph capture\n");
+        return -6;
+    }
     g_host_staged = 0;
+    g_stage_fail = 0;
+    ld_clear_failed();
     return 0;
 }
 
 /* for host-staged calls wait and report; device calls stay asynchronous */
 static int api_end(void) {
     int r;
+    if (g_stage_fail) {
+        ld_sync();
+        ld_clear_info();
+        g_stage_fail = 0;
+        return -2;
+    }
     if (g_host_staged) {
         ld_sync();
         r = ld_get_info();
@@ -70,14 +97,16 @@ static int api_end(void) {
     return r;
 }
 
-#define BEGIN() do { if (api_begin() != 0) return -1; } while (0)
+#define BEGIN() do { const int b_ = api_begin(); if (b_ != 0) return b_; } while (0)
 
 /* ---- factorization drivers --------------------------------------------- */
 
 int dgeqr3(int m, int n, double *A, int lda, double *T, int ldt) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n);
-    dv_dgeqr3(m, n, a.dev, lda, t.dev, ldt);
+    if (!g_stage_fail) {
+        dv_dgeqr3(m, n, a.dev, lda, t.dev, ldt);
+    }
     stage_out(&a, 1); stage_out(&t, 1);
     return api_end();
 }
@@ -88,7 +117,9 @@ int lila_dgeqrf_w03_recursive(int *lila_param, int m, int n, int i, int mt, doub
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
     stage_t q = stage_in(Q, ldq, m, i + n);
-    dv_dgeqrf_w03_recursive(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq);
+    if (!g_stage_fail) {
+        dv_dgeqrf_w03_recursive(lila_param, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
     return api_end();
 }
@@ -100,8 +131,10 @@ int lila_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, i
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
     stage_t q = stage_in(Q, ldq, m, i + n);
-    dv_dgeqrf_w03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev,
-                         ldq);
+    if (!g_stage_fail) {
+        dv_dgeqrf_w03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev,
+                             ldq);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
     return api_end();
 }
@@ -112,7 +145,9 @@ int lila_dgeqrf_w03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, i
         BEGIN();                                                                            \
         stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);           \
         stage_t q = stage_in(Q, ldq, m, i + n);                                             \
-        CALL;                                                                               \
+        if (!g_stage_fail) {                                                                \
+            CALL;                                                                               \
+        }                                                                                   \
         stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);                               \
         return api_end();                                                                   \
     }
@@ -150,7 +185,9 @@ int lila_dormqrf_w03(int m, int n, int k, int i, int j, int mt, double *A, int l
     BEGIN();
     int cols = (i + k > j + n) ? i + k : j + n;
     stage_t a = stage_in(A, lda, m, cols), t = stage_in(T, ldt, ldt, i + k);
-    dv_dormqrf_w03(m, n, k, i, j, mt, a.dev, lda, t.dev, ldt);
+    if (!g_stage_fail) {
+        dv_dormqrf_w03(m, n, k, i, j, mt, a.dev, lda, t.dev, ldt);
+    }
     stage_out(&a, 1); stage_out(&t, 0);
     return api_end();
 }
@@ -161,7 +198,9 @@ int lila_dormqrf_z03(int m, int n, int k, int i, int j, int mt, double *A, int l
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + k), t = stage_in(T, ldt, ldt, i + k);
     stage_t b = stage_in(B, ldb, m, j + n);
-    dv_dormqrf_z03(m, n, k, i, j, mt, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    if (!g_stage_fail) {
+        dv_dormqrf_z03(m, n, k, i, j, mt, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    }
     stage_out(&a, 0); stage_out(&t, 0); stage_out(&b, 1);
     return api_end();
 }
@@ -172,7 +211,9 @@ int lila_dormqrbz_w03(int m, int n, int k, int i, int j, int mt, double *A, int 
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + k), t = stage_in(T, ldt, ldt, i + k);
     stage_t q = stage_in(Q, ldq, m, j + n);
-    dv_dormqrbz_w03(m, n, k, i, j, mt, a.dev, lda, q.dev, ldq, t.dev, ldt);
+    if (!g_stage_fail) {
+        dv_dormqrbz_w03(m, n, k, i, j, mt, a.dev, lda, q.dev, ldq, t.dev, ldt);
+    }
     stage_out(&a, 0); stage_out(&t, 0); stage_out(&q, 1);
     return api_end();
 }
@@ -181,7 +222,9 @@ int lila_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int l
                             int ldt) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
-    dv_dlarft_connect_w03(m, n, i, j, mt, a.dev, lda, t.dev, ldt);
+    if (!g_stage_fail) {
+        dv_dlarft_connect_w03(m, n, i, j, mt, a.dev, lda, t.dev, ldt);
+    }
     stage_out(&a, 0); stage_out(&t, 1);
     return api_end();
 }
@@ -191,7 +234,9 @@ int lila_dlarft_w03(int m, int n, int i, int mt, double *A, int lda, double *T, 
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
     stage_t ta = stage_in(tau, n, n, 1);
-    dv_dlarft_w03(m, n, i, mt, a.dev, lda, t.dev, ldt, ta.dev);
+    if (!g_stage_fail) {
+        dv_dlarft_w03(m, n, i, mt, a.dev, lda, t.dev, ldt, ta.dev);
+    }
     stage_out(&a, 0); stage_out(&t, 1); stage_out(&ta, 0);
     return api_end();
 }
@@ -200,7 +245,9 @@ int lila_dT2tau_w03(int m, int n, int i, int mt, double *T, int ldt, double *tau
     (void)m;
     BEGIN();
     stage_t t = stage_in(T, ldt, ldt, i + n), ta = stage_in(tau, n, n, 1);
-    ld_T2tau(n, i, mt, t.dev, ldt, ta.dev);
+    if (!g_stage_fail) {
+        ld_T2tau(n, i, mt, t.dev, ldt, ta.dev);
+    }
     stage_out(&t, 0); stage_out(&ta, 1);
     return api_end();
 }
@@ -209,7 +256,9 @@ int lila_dV2tau_w03(int m, int n, int i, int mt, double *A, int lda, double *tau
     (void)mt;
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), ta = stage_in(tau, n, n, 1);
-    ld_V2tau(m, n, i, a.dev, lda, ta.dev);
+    if (!g_stage_fail) {
+        ld_V2tau(m, n, i, a.dev, lda, ta.dev);
+    }
     stage_out(&a, 0); stage_out(&ta, 1);
     return api_end();
 }
@@ -222,7 +271,9 @@ int lila_dV2tau_w03(int m, int n, int i, int mt, double *A, int lda, double *tau
         (void)work; (void)lwork;                                                            \
         BEGIN();                                                                            \
         stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);           \
-        CALL;                                                                               \
+        if (!g_stage_fail) {                                                                \
+            CALL;                                                                               \
+        }                                                                                   \
         stage_out(&a, 1); stage_out(&t, 1);                                                 \
         return api_end();                                                                   \
     }
@@ -254,7 +305,9 @@ int lila_dgeqrf_v03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, i
     (void)work; (void)lwork;
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
-    dv_dgeqrf_v03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt);
+    if (!g_stage_fail) {
+        dv_dgeqrf_v03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt);
+    }
     stage_out(&a, 1); stage_out(&t, 1);
     return api_end();
 }
@@ -266,7 +319,9 @@ int lila_dgeqrf_v03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, i
         BEGIN();                                                                            \
         stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);           \
         stage_t q = stage_in(Q, ldq, m, i + n);                                             \
-        CALL;                                                                               \
+        if (!g_stage_fail) {                                                                \
+            CALL;                                                                               \
+        }                                                                                   \
         stage_out(&a, 0); stage_out(&t, 0); stage_out(&q, 1);                               \
         return api_end();                                                                   \
     }
@@ -300,8 +355,10 @@ int lila_dgeqrf_q03_levelx(int *lila_param, int n_lvl, int i_lvl, int *nb_lvl, i
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
     stage_t q = stage_in(Q, ldq, m, i + n);
-    dv_dgeqrf_q03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev,
-                         ldq);
+    if (!g_stage_fail) {
+        dv_dgeqrf_q03_levelx(lila_param, n_lvl, i_lvl, nb_lvl, m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev,
+                             ldq);
+    }
     stage_out(&a, 0); stage_out(&t, 0); stage_out(&q, 1);
     return api_end();
 }
@@ -312,7 +369,9 @@ int lila_dgeqrf_w03_appendcols(int *lila_param, int m, int k, int n, int mt, dou
     BEGIN();
     stage_t a = stage_in(A, lda, m, k + n), t = stage_in(T, ldt, ldt, k + n);
     stage_t q = stage_in(Q, ldq, m, k + n);
-    dv_dgeqrf_w03_appendcols(lila_param, m, k, n, mt, a.dev, lda, q.dev, ldq, t.dev, ldt);
+    if (!g_stage_fail) {
+        dv_dgeqrf_w03_appendcols(lila_param, m, k, n, mt, a.dev, lda, q.dev, ldq, t.dev, ldt);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
     return api_end();
 }
@@ -323,7 +382,9 @@ int lila_dgeqrf_LAPACK_appendcols(int m, int k, int n, int mt, double *A, int ld
     BEGIN();
     stage_t a = stage_in(A, lda, m, k + n), t = stage_in(T, 1, 1, k + n);
     stage_t q = stage_in(Q, ldq, m, k + n);
-    dv_dgeqrf_LAPACK_appendcols(m, k, n, a.dev, lda, q.dev, ldq, t.dev);
+    if (!g_stage_fail) {
+        dv_dgeqrf_LAPACK_appendcols(m, k, n, a.dev, lda, q.dev, ldq, t.dev);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
     return api_end();
 }
@@ -334,7 +395,9 @@ int lila_dgeqrf_ker_recursive(int *lila_param, int m, int n, int i, int mt, doub
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, 1, 1, i + n);
     stage_t q = stage_in(Q, ldq, m, i + n);
-    dv_dgeqrf_ker_recursive(lila_param, m, n, i, a.dev, lda, t.dev, q.dev, ldq);
+    if (!g_stage_fail) {
+        dv_dgeqrf_ker_recursive(lila_param, m, n, i, a.dev, lda, t.dev, q.dev, ldq);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
     return api_end();
 }
@@ -345,7 +408,9 @@ int lila_dormqrf_w00(int m, int n, int k, int i, int j, int mt, double *A, int l
     BEGIN();
     int cols = (j + n > i + k) ? j + n : i + k;
     stage_t a = stage_in(A, lda, m, cols);
-    dv_dormqrf_tau_from_v(m, n, k, i, a.dev, lda, a.dev + i + (size_t)j * lda, lda);
+    if (!g_stage_fail) {
+        dv_dormqrf_tau_from_v(m, n, k, i, a.dev, lda, a.dev + i + (size_t)j * lda, lda);
+    }
     stage_out(&a, 1);
     return api_end();
 }
@@ -355,7 +420,9 @@ int lila_dormqrf_z00(int m, int n, int k, int i, int j, int mt, double *A, int l
     (void)mt; (void)T; (void)ldt; (void)work; (void)lwork;
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + k), b = stage_in(B, ldb, m, j + n);
-    dv_dormqrf_tau_from_v(m, n, k, i, a.dev, lda, b.dev + i + (size_t)j * ldb, ldb);
+    if (!g_stage_fail) {
+        dv_dormqrf_tau_from_v(m, n, k, i, a.dev, lda, b.dev + i + (size_t)j * ldb, ldb);
+    }
     stage_out(&a, 0); stage_out(&b, 1);
     return api_end();
 }
@@ -366,7 +433,9 @@ int lila_dormqrf_z02(int m, int n, int k, int i, int j, int mt, double *A, int l
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + k), t = stage_in(T, ldt, i + k, i + k);
     stage_t b = stage_in(B, ldb, m, j + n);
-    dv_dormqrf_z02(m, n, k, i, j, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    if (!g_stage_fail) {
+        dv_dormqrf_z02(m, n, k, i, j, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    }
     stage_out(&a, 0); stage_out(&t, 0); stage_out(&b, 1);
     return api_end();
 }
@@ -385,7 +454,9 @@ int lila_dgeqrf_qr2_recursive2(int *lila_param, int m, int n, int i, int mt, int
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
     stage_t q = stage_in(Q, ldq, m, i + n);
-    dv_dgeqrf_qr2_recursive2(lila_param, m, n, i, mt, nb1, nb2, a.dev, lda, t.dev, ldt, q.dev, ldq);
+    if (!g_stage_fail) {
+        dv_dgeqrf_qr2_recursive2(lila_param, m, n, i, mt, nb1, nb2, a.dev, lda, t.dev, ldt, q.dev, ldq);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
     return api_end();
 }
@@ -398,7 +469,9 @@ int lila_dorgh2_3(int m, int n, double *A, int lda, double *T, int ldt, double *
     BEGIN();
     stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n), q = stage_in(Q, ldq, m, n);
     stage_t s = stage_in(S, n, n, 1);
-    dv_dorgh2_3(m, n, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    if (!g_stage_fail) {
+        dv_dorgh2_3(m, n, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1); stage_out(&s, 1);
     return api_end();
 }
@@ -409,7 +482,9 @@ int lila_dorgh2(int m, int n, int i, int j, int mt, double *A, int lda, double *
     BEGIN();
     stage_t a = stage_in(A, lda, m, j + n), t = stage_in(T, ldt, ldt, j + n);
     stage_t q = stage_in(Q, ldq, m, j + n), s = stage_in(S, j + n, j + n, 1);
-    dv_dorgh2(m, n, i, j, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    if (!g_stage_fail) {
+        dv_dorgh2(m, n, i, j, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1); stage_out(&s, 1);
     return api_end();
 }
@@ -420,7 +495,9 @@ int lila_ormhr2_w03_hr(int m, int n, int i, int j, int l, int mt, double *A, int
     BEGIN();
     stage_t a = stage_in(A, lda, m, j + n), t = stage_in(T, ldt, ldt, j + n);
     stage_t q = stage_in(Q, ldq, m, j + n), s = stage_in(S, j + n, j + n, 1);
-    dv_ormhr2_w03_hr(m, n, i, j, l, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    if (!g_stage_fail) {
+        dv_ormhr2_w03_hr(m, n, i, j, l, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, s.dev);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1); stage_out(&s, 1);
     return api_end();
 }
@@ -431,22 +508,24 @@ int lila_dorghr(int m, int n, int i, int mt, double *A, int lda, double *T, int 
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), t = stage_in(T, ldt, ldt, i + n);
     stage_t q = stage_in(Q, ldq, m, i + n);
-    size_t mark = ld_ws_mark();
-    double *Sd = (double *)ld_ws_alloc(sizeof(double) * (size_t)(i + n));
-    dv_dorghr(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, Sd);
-    if (S) {
-        double *hs = (double *)malloc(sizeof(double) * (size_t)(i + n));
-        ld_memcpy_d2h(hs, Sd, sizeof(double) * (size_t)(i + n));
-        int dev_s = ld_is_device_ptr(S);
-        int *hi = dev_s ? (int *)malloc(sizeof(int) * (size_t)(i + n)) : S;
-        for (int k = i; k < i + n; k++) hi[k] = (int)hs[k];
-        if (dev_s) {
-            ld_memcpy_h2d(S + i, hi + i, sizeof(int) * (size_t)n);
-            free(hi);
+    if (!g_stage_fail) {
+        size_t mark = ld_ws_mark();
+        double *Sd = (double *)ld_ws_alloc(sizeof(double) * (size_t)(i + n));
+        dv_dorghr(m, n, i, mt, a.dev, lda, t.dev, ldt, q.dev, ldq, Sd);
+        if (S) {
+            double *hs = (double *)malloc(sizeof(double) * (size_t)(i + n));
+            ld_memcpy_d2h(hs, Sd, sizeof(double) * (size_t)(i + n));
+            int dev_s = ld_is_device_ptr(S);
+            int *hi = dev_s ? (int *)malloc(sizeof(int) * (size_t)(i + n)) : S;
+            for (int k = i; k < i + n; k++) hi[k] = (int)hs[k];
+            if (dev_s) {
+                ld_memcpy_h2d(S + i, hi + i, sizeof(int) * (size_t)n);
+                free(hi);
+            }
+            free(hs);
         }
-        free(hs);
+        ld_ws_release(mark);
     }
-    ld_ws_release(mark);
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&q, 1);
     return api_end();
 }
@@ -457,7 +536,9 @@ int lila_dgetrf_b(int m, int n, int i, int mt, double *A, int lda, double *Q, in
     BEGIN();
     stage_t a = stage_in(A, lda, m, i + n), q = stage_in(Q, ldq, m, i + n);
     stage_t u = stage_in(work, n, n, n);
-    dv_dgetrf_b(m, n, i, mt, a.dev, lda, q.dev, ldq, u.dev);
+    if (!g_stage_fail) {
+        dv_dgetrf_b(m, n, i, mt, a.dev, lda, q.dev, ldq, u.dev);
+    }
     stage_out(&a, 1); stage_out(&q, 1); stage_out(&u, 1);
     return api_end();
 }
@@ -469,7 +550,9 @@ int qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R
     stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n);
     const int alias = (R == A);
     stage_t r = alias ? a : stage_in(R, ldr, n, n);
-    dv_qr2_dgeqr3_R(m, n, a.dev, lda, t.dev, ldt, alias ? a.dev : r.dev, alias ? lda : ldr);
+    if (!g_stage_fail) {
+        dv_qr2_dgeqr3_R(m, n, a.dev, lda, t.dev, ldt, alias ? a.dev : r.dev, alias ? lda : ldr);
+    }
     stage_out(&a, 1); stage_out(&t, 1);
     if (!alias) stage_out(&r, 1);
     return api_end();
@@ -478,7 +561,9 @@ int qr2_dgeqr3_R(int m, int n, double *A, int lda, double *T, int ldt, double *R
 int qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *tau) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n), u = stage_in(tau, n, n, 1);
-    dv_qr2_dlarft3(m, n, a.dev, lda, t.dev, ldt, u.dev);
+    if (!g_stage_fail) {
+        dv_qr2_dlarft3(m, n, a.dev, lda, t.dev, ldt, u.dev);
+    }
     stage_out(&a, 0); stage_out(&t, 1); stage_out(&u, 0);
     return api_end();
 }
@@ -486,7 +571,9 @@ int qr2_dlarft3(int m, int n, double *A, int lda, double *T, int ldt, double *ta
 int qr2_dorgqr3(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
     BEGIN();
     stage_t q = stage_in(Q, ldq, m, n), t = stage_in(T, ldt, n, n), u = stage_in(tau, n, n, 1);
-    dv_qr2_dorgqr3(m, n, q.dev, ldq, t.dev, ldt, u.dev);
+    if (!g_stage_fail) {
+        dv_qr2_dorgqr3(m, n, q.dev, ldq, t.dev, ldt, u.dev);
+    }
     stage_out(&q, 1); stage_out(&t, 0); stage_out(&u, 0);
     return api_end();
 }
@@ -506,7 +593,9 @@ int qr2_dorgqr3_VT2Q(int m, int n, double *Q, int ldq, double *T, int ldt, doubl
         double *rd = alias ? a.dev : r.dev;                                            \
         const int rl = alias ? lda : ldr;                                              \
         (void)rd; (void)rl;                                                            \
-        CALL;                                                                          \
+        if (!g_stage_fail) {                                                           \
+            CALL;                                                                      \
+        }                                                                              \
         stage_out(&a, 1); stage_out(&t, 1);                                            \
         if (!alias) stage_out(&r, 1);                                                  \
         return api_end();                                                              \
@@ -521,7 +610,9 @@ QR2_R_ENTRY(qr2_dgeqr3_R_UT_ISW, dv_qr2_dgeqr3_R_isw(1, m, n, a.dev, lda, t.dev,
         BEGIN();                                                                          \
         stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n),                   \
                 u = stage_in(tau, n, n, 1);                                               \
-        CALL;                                                                             \
+        if (!g_stage_fail) {                                                              \
+            CALL;                                                                             \
+        }                                                                                 \
         stage_out(&a, 0); stage_out(&t, 1); stage_out(&u, 0);                             \
         return api_end();                                                                 \
     }
@@ -535,7 +626,9 @@ QR2_T_ENTRY(qr2_dlarft3_ISW_V2T_UT, dv_qr2_dlarft3_isw(1, m, n, a.dev, lda, t.de
 int qr2_dorgqr3_UT(int m, int n, double *Q, int ldq, double *T, int ldt, double *tau) {
     BEGIN();
     stage_t q = stage_in(Q, ldq, m, n), t = stage_in(T, ldt, n, n), u = stage_in(tau, n, n, 1);
-    dv_qr2_dorgqr3_UT(m, n, q.dev, ldq, t.dev, ldt, u.dev);
+    if (!g_stage_fail) {
+        dv_qr2_dorgqr3_UT(m, n, q.dev, ldq, t.dev, ldt, u.dev);
+    }
     stage_out(&q, 1); stage_out(&t, 0); stage_out(&u, 0);
     return api_end();
 }
@@ -548,7 +641,9 @@ int qr2_dorgqr3_VS2Q(int m, int n, double *Q, int ldq, double *T, int ldt, doubl
 int qr2_aux_dVT2Q(int m, int n, double *Q, int ldq) {
     BEGIN();
     stage_t q = stage_in(Q, ldq, m, n);
-    dv_qr2_aux_dVT2Q(m, n, q.dev, ldq);
+    if (!g_stage_fail) {
+        dv_qr2_aux_dVT2Q(m, n, q.dev, ldq);
+    }
     stage_out(&q, 1);
     return api_end();
 }
@@ -556,7 +651,9 @@ int qr2_aux_dVT2Q(int m, int n, double *Q, int ldq) {
 int qr2_aux_dVS2Q(int m, int n, double *Q, int ldq) {
     BEGIN();
     stage_t q = stage_in(Q, ldq, m, n);
-    dv_qr2_aux_dVS2Q(m, n, q.dev, ldq);
+    if (!g_stage_fail) {
+        dv_qr2_aux_dVS2Q(m, n, q.dev, ldq);
+    }
     stage_out(&q, 1);
     return api_end();
 }
@@ -564,7 +661,9 @@ int qr2_aux_dVS2Q(int m, int n, double *Q, int ldq) {
 int qr2_aux_dmLUinA(int n, double *A, int lda) {
     BEGIN();
     stage_t a = stage_in(A, lda, n, n);
-    dv_qr2_aux_dmLUinA(n, a.dev, lda);
+    if (!g_stage_fail) {
+        dv_qr2_aux_dmLUinA(n, a.dev, lda);
+    }
     stage_out(&a, 1);
     return api_end();
 }
@@ -575,7 +674,9 @@ int qr2_aux_dmLUinA(int n, double *A, int lda) {
         const int alias = (L == U);                                 \
         stage_t u = stage_in(U, ldu, n, n);                         \
         stage_t l = alias ? u : stage_in(L, ldl, n, n);             \
-        DV(n, alias ? u.dev : l.dev, alias ? ldu : ldl, u.dev, ldu); \
+        if (!g_stage_fail) {                                         \
+            DV(n, alias ? u.dev : l.dev, alias ? ldu : ldl, u.dev, ldu); \
+        }                                                            \
         stage_out(&u, 1);                                           \
         if (!alias) stage_out(&l, 0);                               \
         return api_end();                                           \
@@ -587,7 +688,9 @@ QR2_LU_ENTRY(qr2_aux_dUinvLTinU, dv_qr2_aux_dUinvLTinU)
 int qr2_aux_dApUBTinA(int m, int n, double *A, int lda, double *U, int ldu, double *B, int ldb) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, n), u = stage_in(U, ldu, m, m), b = stage_in(B, ldb, n, m);
-    dv_qr2_aux_dApUBTinA(m, n, a.dev, lda, u.dev, ldu, b.dev, ldb);
+    if (!g_stage_fail) {
+        dv_qr2_aux_dApUBTinA(m, n, a.dev, lda, u.dev, ldu, b.dev, ldb);
+    }
     stage_out(&a, 1); stage_out(&u, 0); stage_out(&b, 0);
     return api_end();
 }
@@ -595,7 +698,9 @@ int qr2_aux_dApUBTinA(int m, int n, double *A, int lda, double *U, int ldu, doub
 int qr2_aux_dV2N(int n, double *T, int ldt) {
     BEGIN();
     stage_t t = stage_in(T, ldt, n, n);
-    dv_qr2_aux_dV2N(n, t.dev, ldt);
+    if (!g_stage_fail) {
+        dv_qr2_aux_dV2N(n, t.dev, ldt);
+    }
     stage_out(&t, 1);
     return api_end();
 }
@@ -603,7 +708,9 @@ int qr2_aux_dV2N(int n, double *T, int ldt) {
 int qr2_aux_dN2T(int n, double *tau, double *T, int ldt) {
     BEGIN();
     stage_t t = stage_in(T, ldt, n, n), u = stage_in(tau, n, n, 1);
-    dv_qr2_aux_dN2T(n, u.dev, t.dev, ldt);
+    if (!g_stage_fail) {
+        dv_qr2_aux_dN2T(n, u.dev, t.dev, ldt);
+    }
     stage_out(&t, 1); stage_out(&u, 0);
     return api_end();
 }
@@ -611,7 +718,9 @@ int qr2_aux_dN2T(int n, double *tau, double *T, int ldt) {
 int qr2_aux_dV2tau(int m, int n, double *A, int lda, double *tau) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, n), u = stage_in(tau, n, n, 1);
-    dv_qr2_aux_dV2tau(m, n, a.dev, lda, u.dev);
+    if (!g_stage_fail) {
+        dv_qr2_aux_dV2tau(m, n, a.dev, lda, u.dev);
+    }
     stage_out(&a, 0); stage_out(&u, 1);
     return api_end();
 }
@@ -625,7 +734,9 @@ int qr2_aux_dlarf_wrapper(char side, int m, int n, double *V, int incv, double t
     BEGIN();
     const int len = (side == 'L') ? m : n;
     stage_t v = stage_in(V, incv, 1, len), c = stage_in(C, ldc, m, n);
-    dv_qr2_aux_dlarf(side, m, n, v.dev, incv, tau, c.dev, ldc);
+    if (!g_stage_fail) {
+        dv_qr2_aux_dlarf(side, m, n, v.dev, incv, tau, c.dev, ldc);
+    }
     stage_out(&v, 0); stage_out(&c, 1);
     return api_end();
 }
@@ -636,7 +747,9 @@ int lila_dgram(int prec, int m, int n, double *A, int lda, double *G, int ldg) {
     if (prec < 0 || prec > 3 || m < 0 || n < 0) return -1;
     BEGIN();
     stage_t a = stage_in(A, lda, m, n), g = stage_in(G, ldg, n, n);
-    dv_gram(prec, m, n, a.dev, lda, g.dev, ldg);
+    if (!g_stage_fail) {
+        dv_gram(prec, m, n, a.dev, lda, g.dev, ldg);
+    }
     stage_out(&a, 0); stage_out(&g, 1);
     return api_end();
 }
@@ -646,7 +759,9 @@ int lila_dgram(int prec, int m, int n, double *A, int lda, double *G, int ldg) {
 int lila_dgels_qr3(int m, int n, int k, double *A, int lda, double *T, int ldt, double *B, int ldb) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, n), t = stage_in(T, ldt, n, n), b = stage_in(B, ldb, m, k);
-    dv_ls_qr3(m, n, k, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    if (!g_stage_fail) {
+        dv_ls_qr3(m, n, k, a.dev, lda, t.dev, ldt, b.dev, ldb);
+    }
     stage_out(&a, 1); stage_out(&t, 1); stage_out(&b, 1);
     return api_end();
 }
@@ -658,7 +773,9 @@ int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, cons
     stage_t a = stage_in((double *)A, lda, ta ? k : m, ta ? m : k);
     stage_t b = stage_in((double *)B, ldb, tb ? n : k, tb ? k : n);
     stage_t c = stage_in(C, ldc, m, n);
-    ld_gemm_simple(ta, tb, m, n, k, alpha, a.dev, lda, LD_GEN, b.dev, ldb, LD_GEN, beta, c.dev, ldc);
+    if (!g_stage_fail) {
+        ld_gemm_simple(ta, tb, m, n, k, alpha, a.dev, lda, LD_GEN, b.dev, ldb, LD_GEN, beta, c.dev, ldc);
+    }
     stage_out(&a, 0); stage_out(&b, 0); stage_out(&c, 1);
     return api_end();
 }
@@ -666,7 +783,9 @@ int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, cons
 int lila_fill_uniform(unsigned long long seed, int m, int n, double *A, int lda, double lo, double hi) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, n);
-    ld_fill_uniform(seed, m, n, a.dev, lda, lo, hi);
+    if (!g_stage_fail) {
+        ld_fill_uniform(seed, m, n, a.dev, lda, lo, hi);
+    }
     stage_out(&a, 1);
     return api_end();
 }
@@ -674,7 +793,9 @@ int lila_fill_uniform(unsigned long long seed, int m, int n, double *A, int lda,
 int lila_fill_normal(unsigned long long seed, int m, int n, double *A, int lda) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, n);
-    ld_fill_normal(seed, m, n, a.dev, lda);
+    if (!g_stage_fail) {
+        ld_fill_normal(seed, m, n, a.dev, lda);
+    }
     stage_out(&a, 1);
     return api_end();
 }
@@ -742,22 +863,26 @@ int lila_dgels_batched(int nprob, const int *m, const int *n, const int *nrhs, d
     double **db = (double **)malloc(sizeof(double *) * (size_t)nprob);
     int r = -1;
     if (!sa || !sb || !da || !db) goto done;
-    for (int p = 0; p < nprob; p++) {
+    /* stage every problem; a failed staging allocation launches nothing */
+    for (int p = 0; p < nprob && !g_stage_fail; p++) {
         sa[p] = stage_in(A[p], lda[p], m[p], n[p]);
         sb[p] = stage_in(B[p], ldb[p], m[p], nrhs[p]);
         da[p] = sa[p].dev;
         db[p] = sb[p].dev;
     }
-    ld_batch_plan *pl = ld_batch_plan_create(nprob, m, n, nrhs, da, lda, db, ldb);
-    if (pl) {
-        r = ld_batch_plan_run(pl, 0);
-        ld_batch_plan_destroy(pl);  /* waits for the launch */
+    if (!g_stage_fail) {
+        ld_batch_plan *pl = ld_batch_plan_create(nprob, m, n, nrhs, da, lda, db, ldb);
+        if (pl) {
+            r = ld_batch_plan_run(pl, 0);
+            ld_batch_plan_destroy(pl);  /* waits for the launch */
+        }
     }
     for (int p = 0; p < nprob; p++) {
         stage_out(&sa[p], r == 0);
         stage_out(&sb[p], r == 0);
     }
-    if (r == 0) r = api_end();
+    if (g_stage_fail) r = api_end();
+    else if (r == 0) r = api_end();
 done:
     free(sa); free(sb); free(da); free(db);
     return r;

@@ -520,6 +520,16 @@ int dv_dgeqrf_mixed(const int *p, int m, int n, int i, int mt, double *A, int ld
         R = R2;
         R2 = t;
     }
+    /* A (rank deficient, or kappa far beyond 1e8) that makes the low-
+     * precision R0 singular or a Cholesky factor break down is factored by
+     * the FP64 recursion instead, like the reference would: A is untouched
+     * up to here, and T and Q are outputs only */
+    if (ld_check_device_flags() > 0) {
+        ld_clear_info();
+        ld_ws_release(mark);
+        int p64[4] = {0, p[1], p[2], p[3]};
+        return dv_dgeqrf_w03_recursive(p64, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    }
     /* Householder reconstruction of (Q, R) in the mt layout */
     ld_lacpy('U', n, n, R, n, Aii, lda);
     dv_dorghr(m, n, i, mt, A, lda, T, ldt, Q, ldq, Sd);

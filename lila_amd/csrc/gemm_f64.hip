@@ -451,6 +451,7 @@ static void dispatch_tile(const ld_gemm_t *g, int bm, int bn, int splits, int kc
 }
 
 extern "C" void ld_gemm(const ld_gemm_t *g) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (g->m <= 0 || g->n <= 0) return;
   hipStream_t st = (hipStream_t)ld_stream();
   if (g->k <= 0 || g->alpha == 0.0) {
@@ -489,6 +490,7 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
 // two GEMMs).  beta, C, zero_rows and c_upper of g are not used.
 extern "C" int ld_gemm_tmul(const ld_gemm_t *g, const double *T, int ldt, int transT, double *W,
                             int ldw) {
+  if (ld_failed()) return 0;  // an earlier allocation of this call failed
   // (m up to 128 measured no faster: two summing groups per row, 130 KB of LDS)
   if (g->m <= 0 || g->n <= 0 || g->k <= 0 || g->alpha == 0.0 || g->m > 64 || ld_tune()->gemm_lowp)
     return 1;

@@ -304,9 +304,11 @@ static int gram_launch(int f64, int m, int n, const double *A, int lda, double *
 }
 
 extern "C" int ld_gram_bf16x3(int m, int n, const double *A, int lda, double *G, int ldg) {
+  if (ld_failed()) return 0;  // an earlier allocation of this call failed
   return gram_launch(0, m, n, A, lda, G, ldg);
 }
 
 extern "C" int ld_gram_f64(int m, int n, const double *A, int lda, double *G, int ldg) {
+  if (ld_failed()) return 0;  // an earlier allocation of this call failed
   return gram_launch(1, m, n, A, lda, G, ldg);
 }

@@ -138,7 +138,12 @@ void ld_lane_fork(void);
 void ld_lane_join(void);
 int  ld_is_device_ptr(const void *p);
 void ld_ptr_diag(const void *p, int *err, int *type);
-void *ld_ws_alloc(size_t bytes);  /* bump allocation from the arena */
+void *ld_ws_alloc(size_t bytes);  /* bump allocation from the arena; NULL (and
+                                     ld_failed() true) when no chunk can be had */
+int  ld_stream_capturing(void);  /* the library stream is being captured */
+int  ld_failed(void);             /* an allocation of the current call failed:
+                                     launchers and copies return at once */
+void ld_clear_failed(void);
 size_t ld_ws_mark(void);
 void ld_ws_release(size_t mark);
 int  ld_ws_reserve(size_t bytes); /* ensure one free chunk of this size */

@@ -371,6 +371,7 @@ extern "C" {
 
 void ld_trsm(char side, char uplo, char trans, char diag, int m, int n, double alpha,
              const double *A, int lda, double *B, int ldb) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (m <= 0 || n <= 0) return;
   if (alpha != 1.0)
     hipLaunchKernelGGL(scale_kernel, dim3((m + 255) / 256, n), dim3(256), 0, S(), m, n, alpha, B, ldb);
@@ -392,6 +393,7 @@ void ld_trsm(char side, char uplo, char trans, char diag, int m, int n, double a
 
 void ld_trmm(char side, char uplo, char trans, char diag, int m, int n, double alpha,
              const double *A, int lda, double *B, int ldb) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (m <= 0 || n <= 0) return;
   int s = (uplo == 'U') ? (diag == 'U' ? LD_UUNIT : LD_UPPER) : (diag == 'U' ? LD_VUNIT : LD_LOWER);
   int tA = (trans == 'T') ? 1 : 0;
@@ -404,16 +406,19 @@ void ld_trmm(char side, char uplo, char trans, char diag, int m, int n, double a
 }
 
 void ld_syrk_ut(int n, int k, double alpha, const double *A, int lda, double beta, double *C, int ldc) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   gemm(1, 0, n, n, k, alpha, A, lda, LD_GEN, A, lda, LD_GEN, beta, C, ldc, 1);
 }
 
 void ld_potrf_u(int n, double *A, int lda) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   potrf_rec(n, A, lda, 0, nullptr);
 }
 
 void ld_larft(int ml, int k, const double *V, int ldv, const double *tau, double *T, int ldt) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (k <= 0) return;
   if (k <= 32) {
     size_t mark = ld_ws_mark();
@@ -439,6 +444,7 @@ void ld_larft(int ml, int k, const double *V, int ldv, const double *tau, double
 }
 
 void ld_n2t(int n, const double *tau, double *T, int ldt) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   if (n <= 32) {
     size_t mark = ld_ws_mark();
@@ -483,6 +489,7 @@ static void lu_sign_panel(int mr, int nc, double *A, int lda, double *Sg, int ru
 }
 
 void ld_lu_sign(int n, double *Tq, int ldt, double *Sg) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   lu_sign_panel(n, n, Tq, ldt, Sg, 0);
 }
@@ -492,6 +499,7 @@ void ld_lu_sign(int n, double *Tq, int ldt, double *Sg) {
 // gets its rows flipped likewise.  Top block factored with the sign rule,
 // rows below by L2 = A2 D U'^{-1}.
 void ld_getrf_b(int ml, int n, double *A, int lda, double *Q, int ldq, double *U) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0 || ml <= 0) return;
   size_t mark = ld_ws_mark();
   double *Sg = (double *)ld_ws_alloc(sizeof(double) * (size_t)n);
@@ -513,6 +521,7 @@ void ld_getrf_b(int ml, int n, double *A, int lda, double *Q, int ldq, double *U
 // for the current lane's stream.  Used by the mixed-precision driver to
 // decide whether a second Cholesky-QR pass is needed.
 double ld_orth_defect(int n, const double *G, int ldg) {
+  if (ld_failed()) return 0.0;  // an earlier allocation of this call failed
   if (n <= 0) return 0.0;
   size_t mark = ld_ws_mark();
   double *part = (double *)ld_ws_alloc(sizeof(double) * ((size_t)n + 1));
@@ -531,6 +540,7 @@ double ld_orth_defect(int n, const double *G, int ldg) {
 // the last ||F x|| with ||x|| = 1.  Every reduction runs in a fixed order, so
 // the estimate does not vary from run to run.  Waits for the stream.
 double ld_sym_defect_norm2_est(int n, const double *G, int ldg, int iters) {
+  if (ld_failed()) return 0.0;  // an earlier allocation of this call failed
   if (n <= 0) return 0.0;
   size_t mark = ld_ws_mark();
   const int nb = (n + 255) / 256;

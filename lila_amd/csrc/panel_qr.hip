@@ -1094,10 +1094,12 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
 
 extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
                             int ldq) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   panel_launch(ld_tune()->panel_mode, ml, nb, A, lda, T, ldt, Q, ldq);
 }
 
 extern "C" void ld_panel_qr_exact(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
                                   int ldq) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   panel_launch(0, ml, nb, A, lda, T, ldt, Q, ldq);
 }

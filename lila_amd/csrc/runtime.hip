@@ -52,6 +52,7 @@ struct State {
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
   ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64};
+  int failed = 0;  // a workspace allocation of the current call failed
 };
 State g;
 }  // namespace
@@ -114,8 +115,22 @@ int ld_init(void) {
 ld_tune_t *ld_tune(void) { return &g.tune; }
 
 void *ld_stream(void) { return (void *)g.stream; }
+// The switch is ordered: the new stream waits for everything enqueued on
+// the old one (library and device-pointer calls are asynchronous, and the
+// workspace arena, the panel scratch and the barrier counters they share
+// are kept safe by stream order only).
 void ld_set_stream(void *s) {
-  g.user_stream = (hipStream_t)s;
+  const hipStream_t ns = (hipStream_t)s;
+  if (g.inited && ns != g.user_stream) {
+    hipEvent_t e = g.ev[g.evi];
+    g.evi = (g.evi + 1) % NEV;
+    HIPCHECK(hipEventRecord(e, g.user_stream));
+    HIPCHECK(hipEventRecord(g.ev[g.evi], g.aux_stream));
+    HIPCHECK(hipStreamWaitEvent(ns, e, 0));
+    HIPCHECK(hipStreamWaitEvent(ns, g.ev[g.evi], 0));
+    g.evi = (g.evi + 1) % NEV;
+  }
+  g.user_stream = ns;
   if (g.lane == 0) g.stream = g.user_stream;
 }
 
@@ -179,6 +194,7 @@ static int ws_new_chunk(Arena &a, size_t bytes) {
   }
   char *p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
     fprintf(stderr, "lila_hip: cannot allocate %zu bytes of workspace\n", bytes);
     ld_set_info(-2);
     return -1;
@@ -201,11 +217,17 @@ int ld_ws_reserve(size_t bytes) {
   return ws_new_chunk(a, want);
 }
 
+// A workspace chunk that cannot be allocated returns NULL and marks the call
+// failed (info -2): from then on every ld_* launcher and copy of the call
+// returns at once, so no kernel sees the NULL, and the API call returns -2.
 void *ld_ws_alloc(size_t bytes) {
   Arena &a = g.ar[g.lane];
   bytes = (bytes + 255) & ~(size_t)255;
-  if (a.nch == 0 && ws_new_chunk(a, bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes) != 0)
-    abort();
+  if (g.failed) return nullptr;
+  if (a.nch == 0 && ws_new_chunk(a, bytes < ((size_t)64 << 20) ? ((size_t)64 << 20) : bytes) != 0) {
+    g.failed = 1;
+    return nullptr;
+  }
   if (a.ch[a.cur].top + bytes > a.ch[a.cur].cap) {
     // move to the next chunk that fits (chunks above cur are empty)
     int c = a.cur + 1;
@@ -213,7 +235,10 @@ void *ld_ws_alloc(size_t bytes) {
     if (c >= a.nch) {
       size_t want = 2 * a.ch[a.nch - 1].cap;
       if (want < bytes) want = bytes;
-      if (ws_new_chunk(a, want) != 0) abort();
+      if (ws_new_chunk(a, want) != 0) {
+        g.failed = 1;
+        return nullptr;
+      }
       c = a.nch - 1;
     }
     a.cur = c;
@@ -260,6 +285,7 @@ void ld_ws_consolidate(void) {
 void *ld_dev_malloc(size_t bytes) {
   void *p = nullptr;
   if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) {
+    (void)hipGetLastError();  // reported through the info slot, not left pending
     ld_set_info(-2);
     return nullptr;
   }
@@ -269,14 +295,29 @@ void ld_dev_free(void *p) {
   if (p) HIPCHECK(hipFree(p));
 }
 void ld_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (bytes) HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g.stream));
 }
 void ld_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (bytes) {
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g.stream));
     HIPCHECK(hipStreamSynchronize(g.stream));
   }
 }
+
+// 1 when the library stream is being captured into a hipGraph
+int ld_stream_capturing(void) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(g.stream, &st) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return st != hipStreamCaptureStatusNone;
+}
+
+int ld_failed(void) { return g.failed; }
+void ld_clear_failed(void) { g.failed = 0; }
 
 void ld_set_info(int info) {
   if (g.info == 0 || info < 0) g.info = info;
@@ -613,57 +654,70 @@ inline dim3 grid2(int m, int n) { return dim3((unsigned)((m + 255) / 256), (unsi
 extern "C" {
 
 void ld_lacpy(int uplo, int m, int n, const double *A, int lda, double *B, int ldb) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (m <= 0 || n <= 0) return;
   hipLaunchKernelGGL(lacpy_kernel, grid2(m, n), dim3(256), 0, g.stream, uplo, m, n, A, lda, B, ldb);
 }
 void ld_laset(int uplo, int m, int n, double off, double dg, double *A, int lda) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (m <= 0 || n <= 0) return;
   hipLaunchKernelGGL(laset_kernel, grid2(m, n), dim3(256), 0, g.stream, uplo, m, n, off, dg, A, lda);
 }
 void ld_add_diag(int n, double *A, int lda, double v) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   hipLaunchKernelGGL(add_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, g.stream, n, A, lda, v);
 }
 void ld_set_diag(int n, const double *v, int incv, int recip, double *A, int lda) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   hipLaunchKernelGGL(set_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, g.stream, n, v, incv, recip,
                      A, lda);
 }
 void ld_sub(int m, int n, const double *W, int ldw, double *C, int ldc) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (m <= 0 || n <= 0) return;
   hipLaunchKernelGGL(sub_kernel, grid2(m, n), dim3(256), 0, g.stream, m, n, W, ldw, C, ldc);
 }
 void ld_flip_cols(int m, int n, double *X, int ldx, const double *S) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (m <= 0 || n <= 0) return;
   hipLaunchKernelGGL(flip_cols_kernel, grid2(m, n), dim3(256), 0, g.stream, m, n, X, ldx, S);
 }
 void ld_flip_rows_upper(int n, double *X, int ldx, const double *S) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   hipLaunchKernelGGL(flip_rows_upper_kernel, grid2(n, n), dim3(256), 0, g.stream, n, X, ldx, S);
 }
 void ld_flip_rows(int nrows, int n, double *X, int ldx, const double *S) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (nrows <= 0 || n <= 0) return;
   hipLaunchKernelGGL(flip_rows_kernel, grid2(nrows, n), dim3(256), 0, g.stream, nrows, n, X, ldx, S);
 }
 void ld_move_lower(int n, double *Tq, int ldt, double *A, int lda) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 1) return;
   hipLaunchKernelGGL(move_lower_kernel, grid2(n, n), dim3(256), 0, g.stream, n, Tq, ldt, A, lda);
 }
 void ld_T2tau(int n, int i, int mt, const double *T, int ldt, double *tau) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   hipLaunchKernelGGL(t2tau_kernel, dim3((n + 255) / 256), dim3(256), 0, g.stream, n, i, mt, T, ldt, tau);
 }
 void ld_V2tau(int m, int n, int i, const double *A, int lda, double *tau) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   if (n <= 0) return;
   hipLaunchKernelGGL(v2tau_kernel, dim3(n), dim3(256), 0, g.stream, m, i, A, lda, tau);
 }
 void ld_fill_uniform(unsigned long long seed, int m, int n, double *A, int ld, double lo, double hi) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   size_t tot = (size_t)m * n;
   if (!tot) return;
   hipLaunchKernelGGL(fill_uniform_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g.stream,
                      (uint64_t)seed, m, n, A, ld, lo, hi);
 }
 void ld_fill_normal(unsigned long long seed, int m, int n, double *A, int ld) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
   size_t tot = (size_t)m * n;
   if (!tot) return;
   hipLaunchKernelGGL(fill_normal_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g.stream,
