@@ -154,6 +154,95 @@ int dv_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int lda
 
 /* ---- panels ------------------------------------------------------------- */
 
+static void reconstruct_block(int ml, int n, double *A, int lda, double *Tq, int ldt, double *Q,
+                              int ldq, int mq, double *S);
+
+/* TSQR of a tall ml x nb panel (nb <= 32), the tall-panel algorithm of
+ * tsqr/dlatsqr.f (slab QRs whose R factors are factored again) arranged as
+ * a tree: every level's slabs are independent QRs (one workgroup each), the
+ * stacked R's form the next level, until one slab is left.  The explicit Q
+ * is built back down the tree (tsqr/dlamtsqr.f's application to [I; 0]),
+ * and Q with the final R is turned into dgeqr3's V, R and T by the
+ * sign-chosen LU reconstruction (lila-c/lila_dorgh2_3.c:8-51).  Householder
+ * arithmetic throughout: stable for any conditioning.  Q, when given,
+ * receives the explicit panel Q. */
+#define TSQR_MAXLEV 16
+static void tsqr_panel(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q, int ldq) {
+    size_t mark = ld_ws_mark();
+    double *X[TSQR_MAXLEV + 1], *Ts[TSQR_MAXLEV];
+    int M[TSQR_MAXLEV + 1], ldx[TSQR_MAXLEV + 1], S[TSQR_MAXLEV];
+    int L = 0;
+    /* the tree works on a copy: A stays untouched for the column loop below */
+    X[0] = wsd((size_t)ml * nb);
+    ldx[0] = ml;
+    M[0] = ml;
+    ld_lacpy('A', ml, nb, A, lda, X[0], ml);
+    for (;;) {
+        S[L] = ld_tsqr_slabs(M[L], nb);
+        Ts[L] = wsd((size_t)S[L] * 32 * 32);
+        M[L + 1] = S[L] * nb;
+        ldx[L + 1] = M[L + 1];
+        X[L + 1] = wsd((size_t)M[L + 1] * nb);
+        ld_tsqr_leaf(M[L], nb, X[L], ldx[L], S[L], Ts[L], X[L + 1], ldx[L + 1]);
+        L++;
+        if (S[L - 1] == 1 || L == TSQR_MAXLEV) break;
+    }
+    /* X[L] (nb x nb) holds the final R; the explicit Q down the levels */
+    double *Qn = NULL;
+    int ldqn = 0;
+    for (int l = L - 1; l >= 0; l--) {
+        double *Qo;
+        int ldo;
+        if (l == 0) {
+            Qo = Q ? Q : wsd((size_t)ml * nb);
+            ldo = Q ? ldq : ml;
+        } else {
+            Qo = wsd((size_t)M[l] * nb);
+            ldo = M[l];
+        }
+        ld_tsqr_expand(M[l], nb, S[l], X[l], ldx[l], Ts[l], Qn, ldqn, Qo, ldo);
+        Qn = Qo;
+        ldqn = ldo;
+    }
+    /* A reflector with tau = 0 (x = 0: dgeqr3 leaves the column alone) or
+     * within 1e-10 of 0 or 2 shows as |Q_kk| within 1e-10 of 1, which the
+     * reconstruction cannot reproduce (it always reflects): such panels take
+     * dgeqr3's column-by-column loop on the untouched A, as the fused
+     * kernels' acceptance test does */
+    {
+        double *dq = wsd((size_t)nb), hq[32];
+        ld_T2tau(nb, 0, nb + 1, Qn, ldqn, dq);
+        ld_memcpy_d2h(hq, dq, sizeof(double) * (size_t)nb);
+        int degenerate = 0;
+        for (int k = 0; k < nb; k++) degenerate |= !(fabs(hq[k]) <= 1.0 - 1e-10);
+        if (degenerate) {
+            ld_ws_release(mark);
+            ld_panel_qr_exact(ml, nb, A, lda, T, ldt, Q, ldq);
+            return;
+        }
+    }
+    /* reconstruction inputs: R on top of A, Q2 below, Q1 in T */
+    ld_lacpy('U', nb, nb, X[L], ldx[L], A, lda);
+    if (ml > nb) ld_lacpy('A', ml - nb, nb, Qn + nb, ldqn, A + nb, lda);
+    ld_lacpy('A', nb, nb, Qn, ldqn, T, ldt);
+    double *Sd = wsd((size_t)nb);
+    reconstruct_block(ml, nb, A, lda, T, ldt, Q, ldq, Q ? ml : 0, Sd);
+    ld_ws_release(mark);
+}
+
+/* one panel of at most 32 columns: the fused kernel, or for panels taller
+ * than its LDS slabs TSQR (column-by-column mode), or the streamed
+ * Cholesky-QR2 kernel with TSQR for the panels it declines (ill-conditioned:
+ * a Cholesky or LU acceptance test failed) */
+static void panel(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q, int ldq) {
+    if (!ld_panel_is_tall(ml, nb)) {
+        ld_panel_qr(ml, nb, A, lda, T, ldt, Q, ldq);
+        return;
+    }
+    if (ld_tune()->panel_mode == 1 && ld_panel_qr_tall_cqr(ml, nb, A, lda, T, ldt, Q, ldq) == 0) return;
+    tsqr_panel(ml, nb, A, lda, T, ldt, Q, ldq);
+}
+
 /* Recursive panel QR (Elmroth-Gustavson), lila-c/dgeqr3.c:3-114: the left
  * half recursively, its reflectors applied to the right half (:44-70), the
  * right half recursively (:75), T12 = -T11 (V1^T V2) T22 (:95-108).  Widths
@@ -161,7 +250,7 @@ int dv_dlarft_connect_w03(int m, int n, int i, int j, int mt, double *A, int lda
 int dv_dgeqr3(int m, int n, double *A, int lda, double *T, int ldt) {
     if (n <= 0) return 0;
     if (n <= LD_PANEL_MAX) {
-        ld_panel_qr(m, n, A, lda, T, ldt, NULL, 0);
+        panel(m, n, A, lda, T, ldt, NULL, 0);
         return 0;
     }
     int n1 = n / 2, n2 = n - n1;
@@ -196,7 +285,7 @@ int dv_dgeqr2_w03_3(int m, int n, int i, int mt, double *A, int lda, double *T, 
     double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
     double *Tki = T + i % mt + (size_t)i * ldt;
     if (n <= LD_PANEL_MAX) {
-        ld_panel_qr(ml, n, Aii, lda, Tki, ldt, Qii, ldq);
+        panel(ml, n, Aii, lda, Tki, ldt, Qii, ldq);
     } else {
         dv_dgeqr3(ml, n, Aii, lda, Tki, ldt);
         panel_q(ml, n, Aii, lda, Tki, ldt, Qii, ldq);

@@ -64,6 +64,19 @@ void ld_sub(int m, int n, const double *W, int ldw, double *C, int ldc); /* C -=
 #define LD_PANEL_MAX 32
 void ld_panel_qr(int ml, int n, double *A, int lda, double *T, int ldt,
                  double *Q, int ldq);
+/* tall panels (taller than the workgroups' LDS slabs hold): */
+int ld_panel_is_tall(int ml, int n);
+/* the streamed Cholesky-QR2 panel without its column-loop fallback: 0 done,
+ * 1 declined (acceptance test failed, panel untouched); waits for it */
+int ld_panel_qr_tall_cqr(int ml, int n, double *A, int lda, double *T, int ldt, double *Q, int ldq);
+/* TSQR: slab count of an M x nb level, the leaf QRs of a level (V, R in
+ * place, T_s at Ts + s*32*32 (ld 32), stacked R's into Y (S nb x nb, ld ldy)),
+ * and the top-down expansion of a level's explicit Q rows (Xq: the level
+ * above's Q, NULL for the identity) */
+int ld_tsqr_slabs(int M, int nb);
+void ld_tsqr_leaf(int M, int nb, double *X, int ldx, int S, double *Ts, double *Y, int ldy);
+void ld_tsqr_expand(int M, int nb, int S, const double *V, int ldv, const double *Ts,
+                    const double *Xq, int ldx, double *Qo, int ldq);
 /* the same, always column-by-column (dgeqr3's own arithmetic) */
 void ld_panel_qr_exact(int ml, int n, double *A, int lda, double *T, int ldt,
                        double *Q, int ldq);

@@ -39,6 +39,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include "lila_dev.h"
 #include "small32.h"
 
@@ -87,7 +88,10 @@ struct PanelArgs {
   int sr;             // streamed panel: rows per LDS sub-slab
   double *scr;        // persistent global scratch (SCRATCH doubles)
   unsigned *cnt;      // [0] arrivals, [1] exits
-  int *err;
+  int *err;           // [0] barrier time-out, [2] a streamed panel declined
+  int decline;        // streamed panel: leave a panel that fails the acceptance
+                      // tests untouched and raise err[2] (no column loop)
+  int batch;          // every workgroup factors a slab of its own (TSQR leaves)
 };
 
 __device__ __forceinline__ void st_coh(double *p, double v) {
@@ -211,7 +215,7 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
                                     // [7] a dot is not finite
   double *lred = scal + 16;         // SOLO: local reduction vector (SLOT)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int g = blockIdx.x;
+  const int g = a.batch ? 0 : blockIdx.x;
   double *gred = a.scr + OFF_RED;
   if (tid == 0) scal[7] = 0.0;
 
@@ -895,6 +899,13 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     __syncthreads();
   }
 
+  if (!flag[0] && a.decline) {
+    // the panel is untouched: report and leave it to the caller's TSQR
+    if (blockIdx.x == 0 && tid == 0)
+      __hip_atomic_store(&a.err[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    exit_protocol(a);
+    return;
+  }
   if (!flag[0]) {
     // the panel is untouched: column-by-column Householder on it in global
     // memory (every workgroup took the same decisions from the same data)
@@ -962,6 +973,249 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   exit_protocol(a);
 }
 
+// ---------------------------------------------------------------------------
+// TSQR (tall panels): a tree of independent slab QRs.  Level L splits its
+// M x nb matrix into S slabs of nearly equal height (slab s: rows
+// [s M / S, (s + 1) M / S), each >= nb); workgroup s factors its slab in LDS
+// with the column-by-column Householder loop (dlarfg convention, scaled norms
+// and dot products), leaving V and R in place and its T at Ts[s]; the R's
+// stacked (S nb x nb) are the next level's matrix.  The explicit Q is then
+// built top down: slab s's rows are [X_s; 0] - V_s T_s V_s1^T X_s, X_s the
+// s-th nb x nb block of the level above's Q (the identity at the top).
+// ---------------------------------------------------------------------------
+constexpr int TSQR_RP = 256;  // slab rows at most
+
+__device__ __forceinline__ int tsqr_row(int s, int S, int M) {
+  return (int)(((long long)s * M) / S);
+}
+
+// Householder QR (dlarfg convention) of the slab in LDS: nloc x nb, column
+// stride RS, nb <= nloc <= PNT.  V below the diagonal and R on and above it
+// in place, tau[j].  Per column the 256 threads split as (column k = t % 32,
+// 32-row chunk t / 32): each sums its chunk of x.x (k == j) or x.a_k (k > j),
+// the chunk partials are added in a fixed order; when max|x| leaves
+// [1e-150, 1e150] or a sum is not finite the pass is repeated on x times a
+// power of two near 1 / max|x| (dnrm2's purpose).  Then one row per thread
+// for the rank-1 update.
+constexpr int TSQR_TAIL = 8 * 34 + 4 * PNB + 16 + 2 * PNB * LDM + PNB * (PNB + 1) + 8;
+
+__device__ void tsqr_slab_factor(double *slab, int RS, int nloc, int nb, double *tail) {
+  double *red = tail;             // 8 chunks x 34 (32 sums, the chunk max)
+  double *dots = red + 8 * 34;    // combined sums
+  double *wv = dots + PNB;        // w_k
+  double *tau = wv + PNB;         // tau_j
+  double *cmax = tau + PNB;       // per-column flag space (unused)
+  double *scal = cmax + PNB;      // [0] beta [1] tau [2] scale [3] dscale [4] max [5] sc [7] bad
+  const int tid = threadIdx.x, kc = tid & 31, ch = tid >> 5;
+  const int rb = ch * 32, re = min(nloc, rb + 32);
+  (void)cmax;
+  for (int j = 0; j < nb; j++) {
+    for (int pass = 0; pass < 2; pass++) {
+      double inv = 1.0;
+      if (pass == 1) {
+        const double mx = scal[4];
+        if (!(mx != 0.0 && (mx < 1e-150 || mx > 1e150 || scal[7] != 0.0))) break;
+        inv = ldexp(1.0, -ilogb(mx));
+      }
+      double acc = 0.0, mx = 0.0;
+      if (kc >= j && kc < nb) {
+        // fixed trip count, predicated: the 64 LDS reads issue back to back
+        const int rlo = max(rb, j + 1);
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+          const int r = rb + i;
+          const bool ok = r >= rlo && r < re;
+          const double x = ok ? slab[r + j * RS] * inv : 0.0;
+          const double y = ok ? slab[r + kc * RS] * (kc == j ? inv : 1.0) : 0.0;
+          acc += x * y;
+          mx = fmax(mx, fabs(x));
+        }
+      }
+      red[ch * 34 + kc] = acc;
+      if (kc == j) red[ch * 34 + 32] = mx;
+      __syncthreads();
+      if (tid < PNB) {
+        double sm = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) sm += red[c * 34 + tid];
+        dots[tid] = sm;
+      }
+      if (tid == PNB) {
+        double m8 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) m8 = fmax(m8, red[c * 34 + 32]);
+        if (pass == 0) scal[4] = m8;
+        scal[5] = (pass == 0) ? 1.0 : ldexp(1.0, ilogb(scal[4]));
+      }
+      __syncthreads();
+      if (pass == 0 && tid == 0) {
+        int bad = 0;
+        for (int k = j; k < nb; k++) bad |= !(fabs(dots[k]) <= 1.7976931348623157e308);
+        scal[7] = bad ? 1.0 : 0.0;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double sc = scal[5];
+      const double alpha = slab[j + j * RS], xnorm = sc * sqrt(dots[j]);
+      double beta = alpha, t = 0.0, scale = 0.0, dscale = 0.0;
+      if (xnorm != 0.0) {
+        const double aa = fabs(alpha);
+        const double w = fmax(aa, xnorm), z = fmin(aa, xnorm);
+        const double nrm = (z == 0.0) ? w : w * sqrt(1.0 + (z / w) * (z / w));
+        beta = -copysign(nrm, alpha);
+        t = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+        dscale = sc / (alpha - beta);  // the dots are of x / sc
+      }
+      scal[0] = beta;
+      scal[1] = t;
+      scal[2] = scale;
+      scal[3] = dscale;
+      tau[j] = t;
+    }
+    __syncthreads();
+    if (tid > j && tid < nb) wv[tid] = scal[1] * (slab[j + tid * RS] + dots[tid] * scal[3]);
+    __syncthreads();
+    const double t = scal[1];
+    if (t != 0.0 && tid < nloc && tid >= j) {
+      const int r = tid;
+      double v = 1.0;
+      if (r > j) {
+        v = slab[r + j * RS] * scal[2];
+        slab[r + j * RS] = v;
+      } else {
+        slab[j + j * RS] = scal[0];
+      }
+#pragma unroll
+      for (int k = 0; k < PNB; k++)
+        if (k > j && k < nb) slab[r + k * RS] -= v * wv[k];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(PNT) void tsqr_leaf_kernel(int M, int nb, double *X, int ldx, int S, int rs,
+                                                        double *Ts, double *Y, int ldy) {
+  extern __shared__ double smem[];
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int r0 = tsqr_row(s, S, M), nloc = tsqr_row(s + 1, S, M) - r0;
+  double *slab = smem;
+  double *tail = smem + slab_doubles(nb, rs);
+  double *Gm = tail + 8 * 34 + 4 * PNB + 16;  // Gram of V (ld LDM)
+  double *Rs = Gm + PNB * LDM;                 // R (ld LDM)
+  double *Tl = Rs + PNB * LDM;                 // T (ld PNB + 1)
+  const double *taus = tail + 8 * 34 + 2 * PNB;
+  double *Xs = X + r0;
+  // one row per thread (nloc <= PNT): all 32 loads in flight before the
+  // first LDS write
+  if (tid < nloc) {
+    double row[PNB];
+#pragma unroll
+    for (int c = 0; c < PNB; c++) row[c] = (c < nb) ? Xs[tid + (size_t)c * ldx] : 0.0;
+#pragma unroll
+    for (int c = 0; c < PNB; c++)
+      if (c < nb) slab[tid + c * rs] = row[c];
+  }
+  __syncthreads();
+  tsqr_slab_factor(slab, rs, nloc, nb, tail);
+  // R aside; the top block becomes V1 (unit lower) for the Gram V^T V
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t % PNB, c = t / PNB;
+    if (r < nb && c < nb) {
+      const double x = slab[r + c * rs];
+      Rs[r * LDM + c] = (r <= c) ? x : 0.0;
+      slab[r + c * rs] = (r > c) ? x : (r == c ? 1.0 : 0.0);
+    }
+  }
+  __syncthreads();
+  slab_gram(slab, rs, nloc, nb, Gm);
+  __syncthreads();
+  // T = dlarft('F', 'C'): T(0:j, j) = -tau_j T(0:j, 0:j) G(0:j, j)
+  for (int t = tid; t < PNB * (PNB + 1); t += PNT) Tl[t] = 0.0;
+  __syncthreads();
+  for (int j = 0; j < nb; j++) {
+    const double tj = taus[j];
+    double v = 0.0;
+    if (tid < j && tj != 0.0)
+      for (int q = tid; q < j; q++) v += Tl[tid * (PNB + 1) + q] * Gm[q * LDM + j];
+    __syncthreads();
+    if (tid < j) Tl[tid * (PNB + 1) + j] = -tj * v;
+    if (tid == j) Tl[j * (PNB + 1) + j] = tj;
+    __syncthreads();
+  }
+  // V and R back to global, T to Ts[s], R to Y
+  if (tid < nloc) {
+    const int lr = tid;
+#pragma unroll
+    for (int c = 0; c < PNB; c++)
+      if (c < nb) Xs[lr + (size_t)c * ldx] = (lr < nb && lr <= c) ? Rs[lr * LDM + c] : slab[lr + c * rs];
+  }
+  double *Tg = Ts + (size_t)s * PNB * PNB;
+  for (int t = tid; t < nb * nb; t += PNT) {
+    const int r = t % nb, c = t / nb;
+    Tg[r + c * PNB] = (r <= c) ? Tl[r * (PNB + 1) + c] : 0.0;
+    Y[(size_t)s * nb + r + (size_t)c * ldy] = Rs[r * LDM + c];
+  }
+}
+
+// Q rows of slab s of the level's M x nb matrix V (factored in place by the
+// leaf kernel): [X_s; 0] - V_s (T_s (V_s1^T X_s)), X_s rows [s nb, s nb + nb)
+// of Xq (the identity when Xq is null), into Qo rows [r0, r1)
+__global__ __launch_bounds__(PNT) void tsqr_expand_kernel(int M, int nb, int S, const double *V, int ldv,
+                                                          const double *Ts, const double *Xq, int ldx,
+                                                          double *Qo, int ldq) {
+  __shared__ double X[PNB][PNB + 1], W[PNB][PNB + 1], Z[PNB][PNB + 1], Tl[PNB][PNB + 1];
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int r0 = tsqr_row(s, S, M), nloc = tsqr_row(s + 1, S, M) - r0;
+  const double *Vs = V + r0;
+  const double *Tg = Ts + (size_t)s * PNB * PNB;
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t % PNB, c = t / PNB;
+    double x = 0.0, v = 0.0, tv = 0.0;
+    if (r < nb && c < nb) {
+      x = Xq ? Xq[(size_t)s * nb + r + (size_t)c * ldx] : (r == c ? 1.0 : 0.0);
+      v = (r > c) ? Vs[r + (size_t)c * ldv] : (r == c ? 1.0 : 0.0);  // V_s1 (unit lower)
+      tv = (r <= c) ? Tg[r + c * PNB] : 0.0;
+    }
+    X[r][c] = x;
+    Z[r][c] = v;
+    Tl[r][c] = tv;
+  }
+  __syncthreads();
+  // W = V_s1^T X
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int p = t % PNB, c = t / PNB;
+    double acc = 0.0;
+    for (int r = p; r < nb; r++) acc += Z[r][p] * X[r][c];
+    W[p][c] = acc;
+  }
+  __syncthreads();
+  // Z = T W (T upper)
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int p = t % PNB, c = t / PNB;
+    double acc = 0.0;
+    for (int q = p; q < nb; q++) acc += Tl[p][q] * W[q][c];
+    Z[p][c] = acc;
+  }
+  __syncthreads();
+  // rows: [X; 0] - V Z, one row per thread, V's row read across its columns
+  for (int r = tid; r < nloc; r += PNT) {
+    double vr[PNB];
+#pragma unroll
+    for (int p = 0; p < PNB; p++)
+      vr[p] = (p < nb && p <= r) ? (p == r ? 1.0 : Vs[r + (size_t)p * ldv]) : 0.0;
+    for (int c = 0; c < nb; c++) {
+      double acc = (r < nb) ? X[r][c] : 0.0;
+#pragma unroll
+      for (int p = 0; p < PNB; p++) acc -= vr[p] * Z[p][c];
+      Qo[(size_t)(r0 + r) + (size_t)c * ldq] = acc;
+    }
+  }
+}
+
+__global__ void clear_flag_kernel(int *f) { *f = 0; }
+
 size_t slab_doubles_h(int nb, int rs) { return (((size_t)nb * rs + 1) & ~(size_t)1); }
 static_assert(STREAM_TAIL > HH_TAIL, "the streamed panel's fallback uses the LDS base as its tail");
 
@@ -976,15 +1230,9 @@ extern "C" size_t ld_panel_scratch_doubles(void) { return SCRATCH; }
 
 // mode: 0 column-by-column Householder, 1 Cholesky-QR2 + reconstruction
 // (with the column-by-column fallback inside)
-static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
-                         int ldq) {
-  if (nb <= 0 || ml <= 0) return;
-  if (nb > PNB) {
-    ld_fatal("panel width above 32 reached the fused panel kernel");
-    return;
-  }
-  // slab rows per workgroup: as many as LDS holds next to the fixed part,
-  // with an odd column stride (LDS bank spread)
+// slabs of a panel launch: G workgroups of rp rows; taller than 256 LDS
+// slabs (rp > rmax) the panel stays in global memory
+static void panel_geom(int mode, int ml, int nb, int *G_, int *rp_, int *rmax_) {
   int rmax = LDS_SLAB_DOUBLES / nb;
   if (!(rmax & 1)) rmax--;
   int target = (mode == 1) ? ld_tune()->panel_rows_cqr : ld_tune()->panel_rows_per_wg;
@@ -995,6 +1243,22 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   int rp = (ml + G - 1) / G;
   if (rp < nb) rp = nb;
   G = (ml + rp - 1) / rp;
+  *G_ = G;
+  *rp_ = rp;
+  *rmax_ = rmax;
+}
+
+static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
+                         int ldq, int decline) {
+  if (nb <= 0 || ml <= 0) return;
+  if (nb > PNB) {
+    ld_fatal("panel width above 32 reached the fused panel kernel");
+    return;
+  }
+  // slab rows per workgroup: as many as LDS holds next to the fixed part,
+  // with an odd column stride (LDS bank spread)
+  int G, rp, rmax;
+  panel_geom(mode, ml, nb, &G, &rp, &rmax);
   // taller than 256 LDS slabs: the column-by-column kernel on the panel in
   // global memory, 256 workgroups
   const bool glob = rp > rmax;
@@ -1046,6 +1310,7 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   PanelArgs pa;
   pa.ml = ml; pa.nb = nb; pa.A = A; pa.lda = lda; pa.T = T; pa.ldt = ldt;
   pa.Q = Q; pa.ldq = ldq; pa.G = G; pa.rp = rp; pa.rs = rs;
+  pa.decline = decline; pa.batch = 0;
   pa.sr = stream ? srs : rp;
   pa.scr = ld_panel_scratch();
   int *flags = ld_device_flags();
@@ -1095,11 +1360,93 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
 extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
                             int ldq) {
   if (ld_failed()) return;  // an earlier allocation of this call failed
-  panel_launch(ld_tune()->panel_mode, ml, nb, A, lda, T, ldt, Q, ldq);
+  panel_launch(ld_tune()->panel_mode, ml, nb, A, lda, T, ldt, Q, ldq, 0);
 }
 
 extern "C" void ld_panel_qr_exact(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
                                   int ldq) {
   if (ld_failed()) return;  // an earlier allocation of this call failed
-  panel_launch(0, ml, nb, A, lda, T, ldt, Q, ldq);
+  panel_launch(0, ml, nb, A, lda, T, ldt, Q, ldq, 0);
+}
+
+// 1 when an ml x nb panel is taller than the workgroups' LDS slabs can hold
+extern "C" int ld_panel_is_tall(int ml, int nb) {
+  if (nb <= 0 || nb > PNB || ml <= 0) return 0;
+  int G, rp, rmax;
+  panel_geom(ld_tune()->panel_mode, ml, nb, &G, &rp, &rmax);
+  return rp > rmax;
+}
+
+// a tall panel by the streamed Cholesky-QR2 kernel without its column-loop
+// fallback: 0 when it factored the panel, 1 when a Cholesky or LU acceptance
+// test failed and the panel was left untouched (waits for the launch)
+extern "C" int ld_panel_qr_tall_cqr(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,
+                                    int ldq) {
+  if (ld_failed()) return 0;
+  panel_launch(1, ml, nb, A, lda, T, ldt, Q, ldq, 1);
+  int *flags = ld_device_flags();
+  int h = 0;
+  hipStream_t st = (hipStream_t)ld_stream();
+  if (hipMemcpyAsync(&h, flags + 2, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipGetLastError();
+    ld_set_info(-1);
+    return 0;
+  }
+  if (h) {
+    hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, st, flags + 2);
+    return 1;
+  }
+  return 0;
+}
+
+// TSQR levels (slab count S of an M x nb level; M >= S nb)
+extern "C" int ld_tsqr_slabs(int M, int nb) {
+  static int rp = 0;
+  if (!rp) {
+    const char *e = getenv("LILA_TSQR_RP");
+    rp = e ? atoi(e) : TSQR_RP;
+    if (rp < 64 || rp > TSQR_RP) rp = TSQR_RP;
+  }
+  int S = (M + rp - 1) / rp;
+  if (S < 1) S = 1;
+  while (S > 1 && M / S < nb) S--;
+  return S;
+}
+
+extern "C" void ld_tsqr_leaf(int M, int nb, double *X, int ldx, int S, double *Ts, double *Y,
+                             int ldy) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
+  if (M <= 0 || nb <= 0 || nb > PNB) return;
+  const int maxrows = (M + S - 1) / S;
+  if (maxrows > PNT || M / S < nb) {
+    ld_fatal("tsqr: slab outside [nb, 256] rows");
+    return;
+  }
+  const int rs = maxrows | 1;
+  static int attr = 0;
+  const size_t shb = sizeof(double) * (slab_doubles_h(nb, rs) + TSQR_TAIL);
+  if (!attr) {
+    if (hipFuncSetAttribute((const void *)tsqr_leaf_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)LDS_DYN_BYTES) != hipSuccess) {
+      (void)hipGetLastError();
+      ld_fatal("tsqr: dynamic LDS refused");
+      return;
+    }
+    attr = 1;
+  }
+  const int ph = ld_prof_begin("tsqr_leaf_kernel");
+  hipLaunchKernelGGL(tsqr_leaf_kernel, dim3(S), dim3(PNT), shb, (hipStream_t)ld_stream(), M, nb, X, ldx,
+                     S, rs, Ts, Y, ldy);
+  ld_prof_end(ph, 2.0 * M * nb * nb, 8.0 * (2.0 * M * nb + 2.0 * S * nb * nb));
+}
+
+extern "C" void ld_tsqr_expand(int M, int nb, int S, const double *V, int ldv, const double *Ts,
+                               const double *Xq, int ldx, double *Qo, int ldq) {
+  if (ld_failed()) return;  // an earlier allocation of this call failed
+  if (M <= 0 || nb <= 0) return;
+  const int ph = ld_prof_begin("tsqr_expand_kernel");
+  hipLaunchKernelGGL(tsqr_expand_kernel, dim3(S), dim3(PNT), 0, (hipStream_t)ld_stream(), M, nb, S, V,
+                     ldv, Ts, Xq, ldx, Qo, ldq);
+  ld_prof_end(ph, 2.0 * M * nb * nb, 8.0 * (2.0 * M * nb));
 }

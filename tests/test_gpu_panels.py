@@ -3,8 +3,13 @@
 Mode 0 runs every panel column by column (dgeqr3's arithmetic); mode 1 (the
 default) runs Cholesky-QR2 + Householder reconstruction in one launch and
 falls back inside the kernel to the column-by-column loop when the Gram is
-ill-conditioned or a reflector is degenerate.  Both must reproduce dgeqr3's
-V, R, T and Q to rounding.  Tolerances are multiples of n * u.
+ill-conditioned or a reflector is degenerate.  Panels taller than the
+workgroups' LDS slabs take TSQR (a tree of slab Householder QRs, the explicit
+Q built back down the tree, then the sign-chosen LU reconstruction) in mode
+0, and in mode 1 whenever the streamed Cholesky-QR2 kernel declines them;
+degenerate reflectors (tau = 0) still take the column loop.  All must
+reproduce dgeqr3's V, R, T and Q to rounding.  Tolerances are multiples of
+n * u.
 """
 import numpy as np
 import pytest
@@ -134,8 +139,8 @@ def test_cqr_falls_back_on_zero_subcolumns(lila, ora, panel_mode):
 @pytest.mark.parametrize("m", [400000])
 def test_streamed_cqr_falls_back_on_ill_conditioned_tall_panel(lila, ora, panel_mode, m):
     """the ill-conditioned panel above, past the LDS capacity: the streamed
-    kernel has only read the panel when its Cholesky test fails, and runs the
-    column-by-column loop on it in global memory"""
+    kernel has only read the panel when its Cholesky test fails, declines it,
+    and the panel is factored by TSQR"""
     say("streamed cqr fallback: ill-conditioned", m)
     panel_mode(1)
     n = 24
@@ -149,7 +154,8 @@ def test_streamed_cqr_falls_back_on_ill_conditioned_tall_panel(lila, ora, panel_
 @pytest.mark.parametrize("m", [400000])
 def test_streamed_cqr_falls_back_on_zero_subcolumns(lila, ora, panel_mode, m):
     """upper-triangular tall panel: tau = 0 everywhere, the streamed kernel's
-    reconstruction test fails and the fallback returns the panel unchanged"""
+    reconstruction test fails, TSQR's Q shows |Q_kk| = 1, and the column
+    loop returns the panel unchanged"""
     say("streamed cqr fallback: tau = 0", m)
     panel_mode(1)
     n = 20
@@ -178,3 +184,33 @@ def test_panel_modes_agree_on_large_recursive_config1(lila, ora, panel_mode):
     Ag, Tg, Qg = outs[1]
     assert ora.call("test_qq_orth_full", m, n, 0, Qg, m) < 50 * n * U64
     assert ora.call("test_qr_repres_1", m, n, 0, A, m, Qg, m, Ag, m) < 50 * n * U64
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("m,n,spread", [(1000000, 32, 1e12), (600000, 17, 1e14), (400000, 32, 1.0)])
+def test_tsqr_tall_graded_panels(lila, ora, panel_mode, mode, m, n, spread):
+    """tall panels whose columns are graded over `spread` (kappa ~ spread):
+    mode 0 runs TSQR directly, mode 1 has the streamed Cholesky-QR2 kernel
+    decline (its Gram cannot resolve the spread) and then TSQR; V, R, T
+    against the oracle's dgeqr3 (Householder, column by column), and through
+    lila_dgeqr2_w03_3 the explicit panel Q against the oracle's"""
+    say("tsqr tall graded", mode, m, n, spread)
+    panel_mode(mode)
+    A = np.asfortranarray(ora.fill_normal(91 + n, m, n) * np.logspace(0, -np.log10(spread), n)[None, :])
+    (Ao, To), (Ag, Tg) = run_dgeqr3(lila, ora, A)
+    tol = 100 * n * U64
+    assert rel(np.tril(Ag, -1), np.tril(Ao, -1)) < tol, "V"
+    # R row by row relative to its diagonal (graded rows)
+    Rg, Ro = np.triu(Ag[:n]), np.triu(Ao[:n])
+    assert np.max(np.abs(Rg - Ro) / np.abs(np.diag(Ro))[:, None]) < tol, "R"
+    assert rel(np.triu(Tg), np.triu(To)) < tol, "T"
+    # explicit panel Q through the dgeqr2 entry (i = 0, one T block)
+    T2 = np.zeros((n, n), order="F")
+    Q2 = np.zeros((m, n), order="F")
+    Ao2, To2, Qo2 = A.copy(order="F"), T2.copy(order="F"), Q2.copy(order="F")
+    ora.call("lila_dgeqr2_w03_3", m, n, 0, n, Ao2, m, To2, n, Qo2, m, None, 0)
+    Ad, Td, Qd = to_dev(A), to_dev(T2), to_dev(Q2)
+    assert lila.call("lila_dgeqr2_w03_3", m, n, 0, n, Ad, m, Td, n, Qd, m, None, 0) == 0
+    torch.cuda.synchronize()
+    assert lila.call("lila_get_info") == 0
+    assert rel(to_host(Qd), Qo2) < tol, "Q"
