@@ -387,19 +387,26 @@ int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, cons
  * ones (best of `reps` runs each), picks the T block height mt from
  * {256, 512, 1024} and the recursion cutoff nx from {32, 64, 128} (both
  * in/out: read as the starting point, written with the choice, which the
- * caller then passes as mt and lila_param[3]), then the GEMM tile shape,
- * split-K occupancy target, small-output tile threshold and panel slab
- * height, which the library keeps for the rest of the process.  `report`
+ * caller then passes as mt and lila_param[3]), then the GEMM tile shapes,
+ * split-K occupancy target, small-output tile thresholds, panel slab
+ * height, panel width nb (dgeqr3's recursion cutoff, from {32, 16, 24}) and
+ * panel algorithm, which the library keeps for the rest of the process.  `report`
  * (may be NULL) receives the trial times and the choice.  Allocates 3 m x n
  * matrices and a 1024 x n T on the device for the duration.  Returns 0, or
  * -1 without a device, on bad sizes or when the allocation fails. */
 int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int len);
+/* Selects the form lila_autotune times: 0 the plain FP64 path (default), 1
+ * the mixed-precision form (lila_param[0] = 1); returns the previous one. */
+int lila_autotune_form(int form);
 
 /* Panel algorithm of the drivers (returns the previous one): 1 (default)
  * Cholesky-QR2 + Householder reconstruction in one fused launch, with the
  * column-by-column kernel as in-kernel fallback; 0 always column-by-column
  * (dgeqr3's arithmetic).  Both give dgeqr3's V, R, T and Q to rounding. */
 int lila_set_panel_mode(int mode);
+/* Panel width nb (8..32, default 32): dgeqr3's recursion hands widths up to
+ * nb to one panel launch.  Returns the previous width, -1 out of range. */
+int lila_set_panel_width(int nb);
 
 /* Cholesky-QR of the hr paths (lila_dgeqrf_w03_mt_hr, lila_dgeqr2_w03_hr
  * and the hr leaves; returns the previous setting, -1 outside 0..3): 0

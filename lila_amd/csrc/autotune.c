@@ -5,9 +5,11 @@
  *   - the T block height mt and the recursion cutoff nx (handed back to the
  *     caller, who owns the T layout and lila_param[3]);
  *   - the GEMM tile shapes (of products with and without a transposed A),
- *     the split-K occupancy target, the small-output tile thresholds and the
- *     slab height of the Cholesky-QR2 panel (kept in the library's tunables
- *     for the rest of the process).
+ *     the split-K occupancy target, the small-output tile thresholds, the
+ *     slab height of the Cholesky-QR2 panel, the panel width nb (dgeqr3's
+ *     recursion cutoff, 16..32) and the panel algorithm (kept in the
+ *     library's tunables for the rest of the process);
+ *   - for the plain FP64 form or, after lila_autotune_form(1), the mixed one.
  * No counterpart in the reference, whose mt, nx and block sizes are chosen
  * by hand on the command line of its test drivers.
  */
@@ -51,7 +53,7 @@ static void logf_(tune_ctx *c, const char *fmt, ...) {
 
 /* best of `reps` factorizations of the pristine A0 (copied into A first) */
 static double time_qr(tune_ctx *c, int reps) {
-    const int p[4] = {0, 1, 1, c->nx};
+    const int p[4] = {ld_tune()->tune_form == 1 ? 1 : 0, 1, 1, c->nx};
     double best = 1e300;
     for (int r = 0; r < reps; r++) {
         ld_lacpy('A', c->m, c->n, c->A0, c->m, c->A, c->m);
@@ -80,6 +82,16 @@ static void sweep(tune_ctx *c, const char *name, int *slot, const int *vals, int
         *slot = prev;
     }
     *slot = keep;
+}
+
+/* which form lila_autotune times: 0 plain FP64, 1 mixed precision
+ * (lila_param[0] = 1); returns the previous choice */
+int lila_autotune_form(int form) {
+    if (ld_init() != 0) return -1;
+    if (form != 0 && form != 1) return -1;
+    const int old = ld_tune()->tune_form;
+    ld_tune()->tune_form = form;
+    return old;
 }
 
 int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int len) {
@@ -163,10 +175,20 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
         static const int v[] = {96, 128, 192, 256};
         sweep(&c, "panel_rows", &tu->panel_rows_cqr, v, 4);
     }
+    {
+        /* panel width nb (dgeqr3's recursion cutoff) and the panel algorithm */
+        if (tu->panel_nb < 8 || tu->panel_nb > 32) tu->panel_nb = 32;
+        static const int v[] = {32, 16, 24};
+        sweep(&c, "panel_nb", &tu->panel_nb, v, 3);
+        static const int pm[] = {1, 0};
+        sweep(&c, "panel_mode", &tu->panel_mode, pm, 2);
+    }
     logf_(&c, "; chose mt %d nx %d tile %dx%d tile_t %dx%d min_wgs %d small_tiles %d "
-              "small_tiles_t %d panel_rows %d: %.3f ms (%d trials)",
+              "small_tiles_t %d panel_rows %d panel_nb %d panel_mode %d (%s form): %.3f ms "
+              "(%d trials)",
           c.mt, c.nx, tu->gemm_bm, tu->gemm_bn, tu->gemm_bm_t, tu->gemm_bn_t, tu->gemm_min_wgs,
-          tu->gemm_small_tiles, tu->gemm_small_tiles_t, tu->panel_rows_cqr, c.best * 1e3, c.trials);
+          tu->gemm_small_tiles, tu->gemm_small_tiles_t, tu->panel_rows_cqr, tu->panel_nb,
+          tu->panel_mode, tu->tune_form == 1 ? "mixed" : "plain", c.best * 1e3, c.trials);
     *mt = c.mt;
     *nx = c.nx;
     ld_dev_free(A0); ld_dev_free(A); ld_dev_free(Q); ld_dev_free(T);

@@ -247,9 +247,16 @@ static void panel(int ml, int nb, double *A, int lda, double *T, int ldt, double
  * half recursively, its reflectors applied to the right half (:44-70), the
  * right half recursively (:75), T12 = -T11 (V1^T V2) T22 (:95-108).  Widths
  * up to 32 go to the fused panel kernel. */
+/* the panel width: widths up to it go to one panel launch (the autotuner's
+ * choice, 8..32) */
+static int panel_width(void) {
+    const int w = ld_tune()->panel_nb;
+    return (w >= 8 && w <= LD_PANEL_MAX) ? w : LD_PANEL_MAX;
+}
+
 int dv_dgeqr3(int m, int n, double *A, int lda, double *T, int ldt) {
     if (n <= 0) return 0;
-    if (n <= LD_PANEL_MAX) {
+    if (n <= panel_width()) {
         panel(m, n, A, lda, T, ldt, NULL, 0);
         return 0;
     }
@@ -284,7 +291,7 @@ int dv_dgeqr2_w03_3(int m, int n, int i, int mt, double *A, int lda, double *T, 
     int ml = m - i;
     double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
     double *Tki = T + i % mt + (size_t)i * ldt;
-    if (n <= LD_PANEL_MAX) {
+    if (n <= panel_width()) {
         panel(ml, n, Aii, lda, Tki, ldt, Qii, ldq);
     } else {
         dv_dgeqr3(ml, n, Aii, lda, Tki, ldt);

@@ -195,6 +195,8 @@ typedef struct {
     int gemm_small_tiles_t;  /* gemm_small_tiles for products with op(A) = A^T */
     int batch_grid;          /* batched least squares: workgroups of the persistent launch
                                 (0: one per CU at the kernel's occupancy) */
+    int panel_nb;            /* panel width: dgeqr3 recursion cutoff (8..32; 0 = 32) */
+    int tune_form;           /* lila_autotune times the plain (0) or mixed (1) form */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

@@ -457,6 +457,15 @@ int lila_set_panel_mode(int mode) {
   return old;
 }
 
+int lila_set_panel_width(int nb) {
+  if (ld_init() != 0) return -1;
+  if (nb < 8 || nb > LD_PANEL_MAX) return -1;
+  const int old = (g.tune.panel_nb >= 8 && g.tune.panel_nb <= LD_PANEL_MAX) ? g.tune.panel_nb
+                                                                              : LD_PANEL_MAX;
+  g.tune.panel_nb = nb;
+  return old;
+}
+
 int lila_set_gram_precision(int prec) {
   if (ld_init() != 0) return -1;
   if (prec < 0 || prec > 3) return -1;
