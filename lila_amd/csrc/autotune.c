@@ -182,13 +182,17 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
         sweep(&c, "panel_nb", &tu->panel_nb, v, 3);
         static const int pm[] = {1, 0};
         sweep(&c, "panel_mode", &tu->panel_mode, pm, 2);
+        /* Q built at every node (the reference's call tree) or after the VRT
+         * recursion by whole T blocks */
+        static const int qa[] = {0, 1};
+        sweep(&c, "q_after", &tu->q_after, qa, 2);
     }
     logf_(&c, "; chose mt %d nx %d tile %dx%d tile_t %dx%d min_wgs %d small_tiles %d "
-              "small_tiles_t %d panel_rows %d panel_nb %d panel_mode %d (%s form): %.3f ms "
-              "(%d trials)",
+              "small_tiles_t %d panel_rows %d panel_nb %d panel_mode %d q_after %d (%s form): "
+              "%.3f ms (%d trials)",
           c.mt, c.nx, tu->gemm_bm, tu->gemm_bn, tu->gemm_bm_t, tu->gemm_bn_t, tu->gemm_min_wgs,
           tu->gemm_small_tiles, tu->gemm_small_tiles_t, tu->panel_rows_cqr, tu->panel_nb,
-          tu->panel_mode, tu->tune_form == 1 ? "mixed" : "plain", c.best * 1e3, c.trials);
+          tu->panel_mode, tu->q_after, tu->tune_form == 1 ? "mixed" : "plain", c.best * 1e3, c.trials);
     *mt = c.mt;
     *nx = c.nx;
     ld_dev_free(A0); ld_dev_free(A); ld_dev_free(Q); ld_dev_free(T);

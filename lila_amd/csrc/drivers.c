@@ -720,9 +720,54 @@ static int w03_rec(const int *p, int m, int n, int i, int mt, double *A, int lda
     return 0;
 }
 
+/* Q(i:m, i:i+n) = H_i ... H_{i+n-1} [I; 0] from V (below A's diagonal) and
+ * the mt-layout T, by T blocks from the last to the first (LAPACK dorgqr's
+ * order): block b's reflectors applied to the columns right of it, which
+ * hold [0; Q] below the block (lila_dormqrbz_w03's step), and block b's own
+ * columns [I; 0] - V_b T_b V_b1^T.  Every block step is two or three GEMMs
+ * as wide as the columns right of the block. */
+static void q_from_vt(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
+                      int ldq) {
+    if (n <= 0) return;
+    const int nblk = (n + mt - 1) / mt + 1;
+    int *b0 = (int *)malloc(sizeof(int) * (size_t)nblk), *wb = (int *)malloc(sizeof(int) * (size_t)nblk);
+    if (!b0 || !wb) {
+        free(b0); free(wb);
+        ld_set_info(-2);
+        return;
+    }
+    int nb = 0, j = i, w = mt - i % mt;
+    if (w > n) w = n;
+    while (w > 0) {
+        b0[nb] = j;
+        wb[nb++] = w;
+        j += w;
+        w = (j + mt >= i + n) ? (i + n - j) : mt;
+    }
+    for (int b = nb - 1; b >= 0; b--) {
+        const int c0 = b0[b], cw = wb[b], r0 = c0 + cw, nr = i + n - r0;
+        if (nr > 0) dv_dormqrbz_w03(m, nr, cw, c0, r0, mt, A, lda, Q, ldq, T, ldt);
+        panel_q(m - c0, cw, A + c0 + (size_t)c0 * lda, lda, T + c0 % mt + (size_t)c0 * ldt, ldt,
+                Q + c0 + (size_t)c0 * ldq, ldq);
+    }
+    free(b0);
+    free(wb);
+}
+
 int dv_dgeqrf_w03_recursive(const int *p, int m, int n, int i, int mt, double *A, int lda, double *T,
                             int ldt, double *Q, int ldq) {
     if (p[0] == 1) return dv_dgeqrf_mixed(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
+    /* The VRT recursion with the reference's leaves (n <= nx) gives the same
+     * V, R and T as the VRTQ call tree; building Q afterwards by whole T
+     * blocks replaces the Q steps at every node and leaf (many narrow
+     * products near the bottom of the recursion) with a few wide ones.  Q is
+     * the same matrix (H_i ... H_{i+n-1} [I; 0]) to rounding. */
+    if (ld_tune()->q_after && ld_lanes() == 1 && (p[1] == 0 || p[1] == 1) && (p[2] == 0 || p[2] == 1)) {
+        const int pv[4] = {0, p[1], p[2], p[3] + 1};
+        dv_dgeqrf_v03_recursive(pv, m, n, i, mt, A, lda, T, ldt);
+        q_from_vt(m, n, i, mt, A, lda, T, ldt, Q, ldq);
+        return 0;
+    }
     w03_depth++;
     w03_rec(p, m, n, i, mt, A, lda, T, ldt, Q, ldq);
     if (--w03_depth == 0 && ld_lanes() > 1) ld_lane_join();

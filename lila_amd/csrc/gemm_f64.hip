@@ -425,9 +425,11 @@ static void gemm_plan(const ld_gemm_t *g, int &bm, int &bn, int &splits, int &kc
   const int target = tu->gemm_min_wgs;
   if (tiles < target && K >= 256) {
     splits = (int)((target + tiles - 1) / tiles);
-    // down to 4 k-steps of BK per split: a long-K product with a one-tile
-    // output then spreads over up to K/64 workgroups
-    int maxs = K / 64;
+    // down to kmin rows of K per split (4 k-steps of BK by default): a
+    // long-K product with a one-tile output then spreads over up to K/kmin
+    // workgroups
+    const int kmin = (tu->gemm_kmin >= BK) ? tu->gemm_kmin : 64;
+    int maxs = K / kmin;
     if (splits > maxs) splits = maxs;
     // cap partial storage at 64 Mi doubles
     long cap = (64L << 20) / ((long)M * N);

@@ -197,6 +197,9 @@ typedef struct {
                                 (0: one per CU at the kernel's occupancy) */
     int panel_nb;            /* panel width: dgeqr3 recursion cutoff (8..32; 0 = 32) */
     int tune_form;           /* lila_autotune times the plain (0) or mixed (1) form */
+    int gemm_kmin;           /* split-K: rows of K per split at least (0 = 64) */
+    int q_after;             /* VRTQ recursion: 1 builds Q after the VRT recursion by whole T
+                                blocks, 0 at every node as the reference's call tree does */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 
