@@ -407,6 +407,11 @@ int lila_set_panel_mode(int mode);
 /* Panel width nb (8..32, default 32): dgeqr3's recursion hands widths up to
  * nb to one panel launch.  Returns the previous width, -1 out of range. */
 int lila_set_panel_width(int nb);
+/* VRTQ recursion (lila_dgeqrf_w03_recursive with hh or LAPACK leaves): 0
+ * (default) builds Q at every node as the reference's call tree does, 1
+ * after the VRT recursion by whole T blocks from the last to the first (the
+ * same Q to rounding).  Returns the previous setting. */
+int lila_set_q_after(int on);
 
 /* Cholesky-QR of the hr paths (lila_dgeqrf_w03_mt_hr, lila_dgeqr2_w03_hr
  * and the hr leaves; returns the previous setting, -1 outside 0..3): 0

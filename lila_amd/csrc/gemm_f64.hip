@@ -399,6 +399,146 @@ void launch_tile(const ld_gemm_t *g, int splits, int kchunk, double *P, hipStrea
               8.0 * (M * K + K * N + (P ? (double)splits * M * N : 2.0 * M * N)));
 }
 
+// ---------------------------------------------------------------------------
+// 128 x 128 tiles with direct-to-LDS operand loads (global_load_lds_dwordx4:
+// 16 bytes per lane go from HBM / L2 straight into LDS, no VGPR staging), for
+// general operands of products op(A) B with op(A) = A or A^T, B not
+// transposed, every dimension a multiple of the tile (128, 128, 16) and 16-byte
+// aligned operands (the dispatcher below splits structured edges and ragged
+// strips off to the kernel above).  Two LDS stages, one barrier per k step.
+// LDS layouts, chosen so the MFMA operand reads are free of bank conflicts:
+//   k-contiguous operands (B; A when transposed): [k/2][128][2], one 16-byte
+//     chunk (k, k + 1) of one column per lane;
+//   m-contiguous A: [k][144], a 1 KiB column per load instruction, the
+//     16-double pad putting k and k + 1 on opposite halves of the banks.
+// 256 threads = 2 x 2 waves of 64 x 64 (4 x 4 MFMA 16x16x4 tiles each).
+// ---------------------------------------------------------------------------
+constexpr int G2_BM = 128, G2_BN = 128, G2_BK = 16;
+constexpr int G2_ASTAGE = 16 * 144;  // doubles per A stage (the transposed layout uses 2048)
+constexpr int G2_BSTAGE = 2048;
+constexpr size_t G2_LDS = sizeof(double) * 2 * (G2_ASTAGE + G2_BSTAGE);
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ void g2_load(const double *gp, double *lp) {
+  __builtin_amdgcn_global_load_lds((const void *)gp, (lds_void *)lp, 16, 0, 0);
+}
+
+template <bool TA>
+__global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, double alpha, double beta,
+                                                         const double *__restrict__ A, int lda,
+                                                         const double *__restrict__ B, int ldb,
+                                                         double *__restrict__ C, int ldc, int kchunk,
+                                                         double *__restrict__ P) {
+  extern __shared__ __attribute__((aligned(16))) double g2s[];
+  double *As = g2s;                   // [2][G2_ASTAGE]
+  double *Bs = g2s + 2 * G2_ASTAGE;   // [2][G2_BSTAGE]
+  int tx = blockIdx.x, ty = blockIdx.y;
+  {
+    // XCD-aware tile order (as in the kernel above)
+    const int gx = gridDim.x, gy = gridDim.y, ntile = gx * gy;
+    if (ntile >= 512 && gx % 8 == 0 && gy % 8 == 0) {
+      const int lin = blockIdx.x + blockIdx.y * gx;
+      const int t = (lin % 8) * (ntile / 8) + lin / 8;
+      const int st = t / 64, w = t % 64, sx = gx / 8;
+      tx = (st % sx) * 8 + w % 8;
+      ty = (st / sx) * 8 + w / 8;
+    }
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int m0 = tx * G2_BM, n0 = ty * G2_BN;
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  const int nsteps = (kend - kbeg) / G2_BK;
+
+  // this wave's four load instructions per operand and stage
+  auto issue = [&](int buf, int k0) {
+    double *as = As + buf * G2_ASTAGE, *bs = Bs + buf * G2_BSTAGE;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int jj = wave * 4 + j;
+      const int kp = jj >> 1, x = (jj & 1) * 64 + lane;
+      g2_load(B + (size_t)(k0 + 2 * kp) + (size_t)(n0 + x) * ldb, bs + jj * 128);
+      if (TA)
+        g2_load(A + (size_t)(k0 + 2 * kp) + (size_t)(m0 + x) * lda, as + jj * 128);
+      else
+        g2_load(A + (size_t)(m0 + 2 * lane) + (size_t)(k0 + jj) * lda, as + jj * 144);
+    }
+  };
+
+  d4_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  const int r = lane & 15, kk = lane >> 4;
+  if (nsteps > 0) issue(0, kbeg);
+  for (int t = 0; t < nsteps; t++) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of step t are in LDS
+    __syncthreads();                // everyone's, and step t - 1's stage is free
+    if (t + 1 < nsteps) issue((t + 1) & 1, kbeg + (t + 1) * G2_BK);
+    const double *as = As + (t & 1) * G2_ASTAGE, *bs = Bs + (t & 1) * G2_BSTAGE;
+    // operand fragments of sub-step s + 1 read from LDS while the 16 MFMAs
+    // of sub-step s issue
+    double af[2][4], bf[2][4];
+    auto frags = [&](int sstep, double (&fa)[4], double (&fb)[4]) {
+      const int k = 4 * sstep + kk;
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        const int m = wm * 64 + a * 16 + r;
+        fa[a] = TA ? as[(k >> 1) * 256 + 2 * m + (k & 1)] : as[k * 144 + m];
+      }
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int n = wn * 64 + b * 16 + r;
+        fb[b] = bs[(k >> 1) * 256 + 2 * n + (k & 1)];
+      }
+    };
+    frags(0, af[0], bf[0]);
+#pragma unroll
+    for (int sstep = 0; sstep < G2_BK / 4; sstep++) {
+      const int cur = sstep & 1;
+      if (sstep + 1 < G2_BK / 4) frags(sstep + 1, af[cur ^ 1], bf[cur ^ 1]);
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = mfma16(bf[cur][b], af[cur][a], acc[a][b]);
+    }
+  }
+
+  // epilogue (every tile is interior): lane holds C(row m0 + .. + r,
+  // column n0 + .. + kk + 4v); by 16-row groups, each group's 16 C loads
+  // issued before its first store (the registers of all 64 would spill)
+  const bool read_c = (P == nullptr) && beta != 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const int gi = m0 + wm * 64 + a * 16 + r;
+    double cv[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = n0 + wn * 64 + b * 16 + kk + 4 * v;
+        cv[b][v] = read_c ? C[(size_t)gi + (size_t)gj * ldc] : 0.0;
+      }
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = n0 + wn * 64 + b * 16 + kk + 4 * v;
+        if (P) {
+          P[(size_t)blockIdx.z * M * N + (size_t)gi + (size_t)gj * M] = acc[a][b][v];
+        } else {
+          double o = alpha * acc[a][b][v];
+          if (read_c) o += beta * cv[b][v];
+          C[(size_t)gi + (size_t)gj * ldc] = o;
+        }
+      }
+  }
+}
+
 }  // namespace
 
 // tile shape and split-K count of a product
@@ -452,8 +592,115 @@ static void dispatch_tile(const ld_gemm_t *g, int bm, int bn, int splits, int kc
 #undef DISPATCH
 }
 
+static void gemm_tiles(const ld_gemm_t *g);
+
+// the 128 x 128 direct-to-LDS kernel on a general product (sa = sb = GEN,
+// no transposed B, 16-byte aligned operands): its 128-aligned interior with
+// K a multiple of 16, the ragged strips and the K tail by the tiled kernel
+static bool g2_eligible(const ld_gemm_t *g) {
+  // (products with op(A) = A^T measured slower on it than on the tiled
+  // kernel with split-K: NN only)
+  return (!g->ta || ld_tune()->gemm_v2 == 2) && g->sa == LD_GEN && g->sb == LD_GEN && !g->tb && !g->c_upper && g->zero_rows == 0 &&
+         !ld_tune()->gemm_lowp && ld_tune()->gemm_v2 && ((uintptr_t)g->A & 15) == 0 &&
+         ((uintptr_t)g->B & 15) == 0 && (g->lda & 1) == 0 && (g->ldb & 1) == 0;
+}
+
+static void gemm2_general(const ld_gemm_t *g) {
+  const int M = g->m, N = g->n, K = g->k;
+  const int Mi = M / G2_BM * G2_BM, Ni = N / G2_BN * G2_BN, Ki = K / G2_BK * G2_BK;
+  const long tiles = (long)(Mi / G2_BM) * (Ni / G2_BN);
+  if (tiles < 128 || Ki == 0) {
+    gemm_tiles(g);
+    return;
+  }
+  hipStream_t st = (hipStream_t)ld_stream();
+  static int attr = 0;
+  if (!attr) {
+    for (const void *f : {(const void *)gemm2_f64_kernel<false>, (const void *)gemm2_f64_kernel<true>}) {
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G2_LDS) != hipSuccess) {
+        (void)hipGetLastError();
+        ld_tune()->gemm_v2 = 0;
+        gemm_tiles(g);
+        return;
+      }
+    }
+    attr = 1;
+  }
+  {
+    const int ph = ld_prof_begin(g->ta ? "gemm2_f64_kernel<true>" : "gemm2_f64_kernel<false>");
+    dim3 grid(Mi / G2_BM, Ni / G2_BN, 1);
+    if (g->ta)
+      hipLaunchKernelGGL(gemm2_f64_kernel<true>, grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha, g->beta,
+                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, Ki, (double *)nullptr);
+    else
+      hipLaunchKernelGGL(gemm2_f64_kernel<false>, grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha, g->beta,
+                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, Ki, (double *)nullptr);
+    ld_prof_end(ph, 2.0 * Mi * Ni * Ki, 8.0 * ((double)Mi * Ki + (double)Ki * Ni + 2.0 * Mi * Ni));
+  }
+  // the K tail onto the interior (beta = 1)
+  if (Ki < K) {
+    ld_gemm_t h = *g;
+    h.m = Mi; h.n = Ni; h.k = K - Ki; h.beta = 1.0;
+    h.A = g->ta ? g->A + Ki : g->A + (size_t)Ki * g->lda;
+    h.B = g->B + Ki;
+    gemm_tiles(&h);
+  }
+  // right strip (all rows), bottom strip
+  if (Ni < N) {
+    ld_gemm_t h = *g;
+    h.n = N - Ni;
+    h.B = g->B + (size_t)Ni * g->ldb;
+    h.C = g->C + (size_t)Ni * g->ldc;
+    gemm_tiles(&h);
+  }
+  if (Mi < M) {
+    ld_gemm_t h = *g;
+    h.m = M - Mi; h.n = Ni;
+    h.A = g->ta ? g->A + (size_t)Mi * g->lda : g->A + Mi;
+    h.C = g->C + Mi;
+    gemm_tiles(&h);
+  }
+}
+
 extern "C" void ld_gemm(const ld_gemm_t *g) {
   if (ld_failed()) return;  // an earlier allocation of this call failed
+  if (g->m <= 0 || g->n <= 0) return;
+  if (g->k > 0 && g->alpha != 0.0 && ld_tune()->gemm_v2 && !ld_tune()->gemm_lowp && !g->tb &&
+      g->sb == LD_GEN && !g->c_upper && (g->sa == LD_GEN || g->sa == LD_VUNIT)) {
+    // the general part of the product (all of it, or what is left of a
+    // Householder V once its unit lower top block is split off: C's first k
+    // rows when op(A) = V, the first m rows of K when op(A) = V^T)
+    ld_gemm_t h = *g, top = *g;
+    bool split = false;
+    if (g->sa == LD_VUNIT) {
+      split = true;
+      if (!g->ta) {
+        const int kv = g->k;
+        if (g->zero_rows > kv || g->m <= kv) split = false;
+        top.m = kv;
+        h.m = g->m - kv; h.A = g->A + kv; h.C = g->C + kv; h.sa = LD_GEN; h.zero_rows = 0;
+      } else {
+        const int kv = g->m;
+        if (g->k <= kv || g->zero_rows > 0) split = false;
+        top.k = kv;
+        h.k = g->k - kv; h.A = g->A + kv; h.B = g->B + kv; h.sa = LD_GEN; h.beta = 1.0;
+      }
+      if (!split) {
+        gemm_tiles(g);
+        return;
+      }
+    }
+    const long tiles = (long)(h.m / G2_BM) * (h.n / G2_BN);
+    if (g2_eligible(&h) && tiles >= 128 && h.k >= G2_BK) {
+      if (split) gemm_tiles(&top);
+      gemm2_general(&h);
+      return;
+    }
+  }
+  gemm_tiles(g);
+}
+
+static void gemm_tiles(const ld_gemm_t *g) {
   if (g->m <= 0 || g->n <= 0) return;
   hipStream_t st = (hipStream_t)ld_stream();
   if (g->k <= 0 || g->alpha == 0.0) {

@@ -198,6 +198,7 @@ typedef struct {
     int panel_nb;            /* panel width: dgeqr3 recursion cutoff (8..32; 0 = 32) */
     int tune_form;           /* lila_autotune times the plain (0) or mixed (1) form */
     int gemm_kmin;           /* split-K: rows of K per split at least (0 = 64) */
+    int gemm_v2;             /* 1: general products on the 128 x 128 direct-to-LDS kernel */
     int q_after;             /* VRTQ recursion: 1 builds Q after the VRT recursion by whole T
                                 blocks, 0 at every node as the reference's call tree does */
 } ld_tune_t;

@@ -51,7 +51,7 @@ struct State {
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
-  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64, 0, 0, 0, 0, 0};
+  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64, 0, 0, 0, 0, 1, 0};
   int failed = 0;  // a workspace allocation of the current call failed
 };
 State g;
@@ -104,6 +104,8 @@ int ld_init(void) {
   if (e) g.tune.gemm_small_tiles = atoi(e);
   e = getenv("LILA_GEMM_SMALL_TILES_T");
   if (e) g.tune.gemm_small_tiles_t = atoi(e);
+  e = getenv("LILA_GEMM_V2");
+  if (e) g.tune.gemm_v2 = atoi(e);
   e = getenv("LILA_Q_AFTER");
   if (e) g.tune.q_after = atoi(e) ? 1 : 0;
   e = getenv("LILA_GEMM_KMIN");
@@ -467,6 +469,13 @@ int lila_set_panel_width(int nb) {
   const int old = (g.tune.panel_nb >= 8 && g.tune.panel_nb <= LD_PANEL_MAX) ? g.tune.panel_nb
                                                                               : LD_PANEL_MAX;
   g.tune.panel_nb = nb;
+  return old;
+}
+
+int lila_set_q_after(int on) {
+  if (ld_init() != 0) return -1;
+  const int old = g.tune.q_after;
+  g.tune.q_after = on ? 1 : 0;
   return old;
 }
 

@@ -64,6 +64,31 @@ def test_dgemm(lila, ora, ta, tb, m, n, k):
     assert np.all(np.abs(got - ref) <= bound + 1e-300)
 
 
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("m,n,k", [(2048, 2048, 512), (2100, 2200, 530), (1024, 8192, 3000),
+                                   (4096, 1152, 16)])
+def test_dgemm_direct_lds_tiles(lila, ora, ta, m, n, k):
+    """shapes large enough for the 128 x 128 direct-to-LDS kernel: its
+    interior, the ragged strips and the K tail (run by the tiled kernel) must
+    add up to the product"""
+    say("dgemm 128x128 direct-to-LDS", ta, m, n, k)
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    A = np.asfortranarray(rng.standard_normal((k, m) if ta == "T" else (m, k)))
+    B = np.asfortranarray(rng.standard_normal((k, n)))
+    C = np.asfortranarray(rng.standard_normal((m, n)))
+    opA = A.T if ta == "T" else A
+    ref = -0.75 * opA @ B + 1.25 * C
+    Cd = to_dev(C)
+    import ctypes
+    r = lila.call("lila_dgemm", ctypes.c_char(ta.encode()), ctypes.c_char(b"N"), m, n, k, -0.75,
+                  to_dev(A), A.shape[0], to_dev(B), k, 1.25, Cd, m)
+    assert r == 0
+    sync_ok(lila)
+    got = to_host(Cd)
+    bound = 8 * (k + 2) * U64 * (0.75 * np.abs(opA) @ np.abs(B) + 1.25 * np.abs(C))
+    assert np.all(np.abs(got - ref) <= bound + 1e-300)
+
+
 # ---------------------------------------------------------------------------
 # panel: dgeqr3 (fused kernel up to 32 columns, recursion above)
 # ---------------------------------------------------------------------------
@@ -152,6 +177,22 @@ def test_recursive(lila, ora, case, leaf, panel):
     A, o, g = run_driver(lila, ora, "recursive", m, n, ii, mt, [0, leaf, panel, nx], lda=lda, ldq=ldq)
     check_vrtq(ora, A, o, g, m, n, ii, mt, lda or m, ldq or m,
                tolf=(2000 if (leaf == 2 or panel == 2) else 100))
+
+
+@pytest.mark.parametrize("leaf,panel", [(1, 1), (0, 1), (1, 0)])
+@pytest.mark.parametrize("case", [CASES[1], CASES[2], CASES[3], CASES[4]])
+def test_recursive_q_after(lila, ora, case, leaf, panel):
+    """Q built after the VRT recursion by whole T blocks (lila_set_q_after):
+    the same V, R, T and Q as the reference's call tree, to rounding"""
+    m, n, ii, mt, nx, lda, ldq = case
+    say("recursive q_after", case, leaf, panel)
+    old = lila.call("lila_set_q_after", 1)
+    try:
+        A, o, g = run_driver(lila, ora, "recursive", m, n, ii, mt, [0, leaf, panel, nx], lda=lda,
+                             ldq=ldq)
+    finally:
+        lila.call("lila_set_q_after", old)
+    check_vrtq(ora, A, o, g, m, n, ii, mt, lda or m, ldq or m)
 
 
 @pytest.mark.parametrize("nb_lvl", [[32], [128], [64, 16]])
