@@ -385,12 +385,13 @@ int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, cons
  * lila_dgeqrf_w03_recursive (VRTQ, FP64, panel dgeqr3, leaf hh) on an m x n
  * Uniform(-1,1) device matrix and, one setting at a time from the current
  * ones (best of `reps` runs each), picks the T block height mt from
- * {256, 512, 1024} and the recursion cutoff nx from {32, 64, 128} (both
+ * {256, 512, 1024} and the recursion cutoff nx from {32, 64, 128, 256} (both
  * in/out: read as the starting point, written with the choice, which the
  * caller then passes as mt and lila_param[3]), then the GEMM tile shapes,
  * split-K occupancy target, small-output tile thresholds, panel slab
- * height, panel width nb (dgeqr3's recursion cutoff, from {32, 16, 24}) and
- * panel algorithm, which the library keeps for the rest of the process.  `report`
+ * height, panel width nb (dgeqr3's recursion cutoff, from {32, 16, 24}),
+ * panel algorithm and where Q is built (lila_set_q_after), which the library
+ * keeps for the rest of the process.  `report`
  * (may be NULL) receives the trial times and the choice.  Allocates 3 m x n
  * matrices and a 1024 x n T on the device for the duration.  Returns 0, or
  * -1 without a device, on bad sizes or when the allocation fails. */
