@@ -379,6 +379,16 @@ void lila_dgels_batched_destroy(lila_batch_plan *plan);
  * (transa/transb 'N' or 'T'); the GEMM every BLAS-3 step above runs on. */
 int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double *A,
                int lda, const double *B, int ldb, double beta, double *C, int ldc);
+/* No counterpart in the reference: the same product emulated on int8 MFMA
+ * by the Chinese-remainder scheme (Ozaki scheme II): rows of op(A) and
+ * columns of B scaled to kb-bit integers (8 <= kb <= 53), their residues
+ * modulo 14-16 coprime moduli <= 256 multiplied exactly in int32, the
+ * integer product recovered by the CRT in FP64 and scaled back.  With
+ * kb = 53 the error is that of an FP64 product norm-wise (relative 2^-53 of
+ * each row's and column's largest entry).  transb must be 'N', k <= 65536;
+ * finite entries only.  Returns -1 otherwise. */
+int lila_dgemm_emu(char transa, char transb, int m, int n, int k, double alpha, const double *A,
+                   int lda, const double *B, int ldb, double beta, double *C, int ldc, int kb);
 
 /* Autotuner (no counterpart in the reference, whose mt, nx and block sizes
  * are set by hand on its test drivers' command lines): times

@@ -780,6 +780,32 @@ int lila_dgemm(char transa, char transb, int m, int n, int k, double alpha, cons
     return api_end();
 }
 
+/* the same product emulated on int8 MFMA (CRT scheme, kb-bit operands;
+ * transb must be 'N'): -1 for a product the emulation does not take */
+int lila_dgemm_emu(char transa, char transb, int m, int n, int k, double alpha, const double *A,
+                   int lda, const double *B, int ldb, double beta, double *C, int ldc, int kb) {
+    int ta = (transa == 'T' || transa == 't'), tb = (transb == 'T' || transb == 't');
+    if (tb || m < 0 || n < 0 || k < 0 || k > 65536 || kb < 8 || kb > 53) return -1;
+    BEGIN();
+    stage_t a = stage_in((double *)A, lda, ta ? k : m, ta ? m : k);
+    stage_t b = stage_in((double *)B, ldb, k, n);
+    stage_t c = stage_in(C, ldc, m, n);
+    int r = 0;
+    if (!g_stage_fail) {
+        ld_gemm_t g;
+        memset(&g, 0, sizeof g);
+        g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+        g.A = a.dev; g.lda = lda; g.ta = ta; g.sa = LD_GEN;
+        g.B = b.dev; g.ldb = ldb; g.tb = 0; g.sb = LD_GEN;
+        g.C = c.dev; g.ldc = ldc;
+        if (k == 0 || alpha == 0.0) ld_gemm(&g);
+        else r = ld_gemm_emu(&g, kb);
+    }
+    stage_out(&a, 0); stage_out(&b, 0); stage_out(&c, r == 0);
+    const int e = api_end();
+    return e != 0 ? e : r;
+}
+
 int lila_fill_uniform(unsigned long long seed, int m, int n, double *A, int lda, double lo, double hi) {
     BEGIN();
     stage_t a = stage_in(A, lda, m, n);

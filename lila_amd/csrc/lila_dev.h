@@ -33,6 +33,11 @@ typedef struct {
 
 /* C = alpha op(A) op(B) + beta C on FP64 MFMA (split-K chosen inside) */
 void ld_gemm(const ld_gemm_t *g);
+/* the same emulated on int8 MFMA by the CRT scheme with kb-bit operands
+ * (gemm_emu.hip): general op(A) = A or A^T, B not transposed, K <= 65536;
+ * 0 when done, -1 when the product is not one it takes */
+int ld_gemm_emu(const ld_gemm_t *g, int kb);
+int ld_emu_moduli(int K, int kb);
 static inline void ld_gemm_simple(int ta, int tb, int m, int n, int k, double alpha,
                                   const double *A, int lda, int sa,
                                   const double *B, int ldb, int sb,
