@@ -407,8 +407,10 @@ void launch_tile(const ld_gemm_t *g, int splits, int kchunk, double *P, hipStrea
 // aligned operands (the dispatcher below splits structured edges and ragged
 // strips off to the kernel above).  Two LDS stages, one barrier per k step.
 // LDS layouts, chosen so the MFMA operand reads are free of bank conflicts:
-//   k-contiguous operands (B; A when transposed): [k/2][128][2], one 16-byte
-//     chunk (k, k + 1) of one column per lane;
+//   k-contiguous operands (B; A when transposed): column x's 16 k values as
+//     8 16-byte chunks (k, k + 1) at x * 16 doubles, chunk c in slot
+//     c ^ ((x >> 1) & 7); a load instruction fills 8 whole columns, 8 lanes
+//     per column reading its 128 contiguous bytes;
 //   m-contiguous A: [k][144], a 1 KiB column per load instruction, the
 //     16-double pad putting k and k + 1 on opposite halves of the banks.
 // 256 threads = 2 x 2 waves of 64 x 64 (4 x 4 MFMA 16x16x4 tiles each).
@@ -452,16 +454,17 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
   const int kend = min(K, kbeg + kchunk);
   const int nsteps = (kend - kbeg) / G2_BK;
 
-  // this wave's four load instructions per operand and stage
+  // this wave's four load instructions per operand and stage; load jj of a
+  // k-contiguous operand fills columns 8 jj .. 8 jj + 7
   auto issue = [&](int buf, int k0) {
     double *as = As + buf * G2_ASTAGE, *bs = Bs + buf * G2_BSTAGE;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int jj = wave * 4 + j;
-      const int kp = jj >> 1, x = (jj & 1) * 64 + lane;
-      g2_load(B + (size_t)(k0 + 2 * kp) + (size_t)(n0 + x) * ldb, bs + jj * 128);
+      const int x = jj * 8 + (lane >> 3), c = (lane & 7) ^ ((x >> 1) & 7);
+      g2_load(B + (size_t)(k0 + 2 * c) + (size_t)(n0 + x) * ldb, bs + jj * 128);
       if (TA)
-        g2_load(A + (size_t)(k0 + 2 * kp) + (size_t)(m0 + x) * lda, as + jj * 128);
+        g2_load(A + (size_t)(k0 + 2 * c) + (size_t)(m0 + x) * lda, as + jj * 128);
       else
         g2_load(A + (size_t)(m0 + 2 * lane) + (size_t)(k0 + jj) * lda, as + jj * 144);
     }
@@ -488,12 +491,12 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
 #pragma unroll
       for (int a = 0; a < 4; a++) {
         const int m = wm * 64 + a * 16 + r;
-        fa[a] = TA ? as[(k >> 1) * 256 + 2 * m + (k & 1)] : as[k * 144 + m];
+        fa[a] = TA ? as[m * 16 + (((k >> 1) ^ ((m >> 1) & 7)) << 1) + (k & 1)] : as[k * 144 + m];
       }
 #pragma unroll
       for (int b = 0; b < 4; b++) {
         const int n = wn * 64 + b * 16 + r;
-        fb[b] = bs[(k >> 1) * 256 + 2 * n + (k & 1)];
+        fb[b] = bs[n * 16 + (((k >> 1) ^ ((n >> 1) & 7)) << 1) + (k & 1)];
       }
     };
     frags(0, af[0], bf[0]);
@@ -598,11 +601,12 @@ static void gemm_tiles(const ld_gemm_t *g);
 // no transposed B, 16-byte aligned operands): its 128-aligned interior with
 // K a multiple of 16, the ragged strips and the K tail by the tiled kernel
 static bool g2_eligible(const ld_gemm_t *g) {
-  // (products with op(A) = A^T measured slower on it than on the tiled
-  // kernel with split-K: NN only)
-  return (!g->ta || ld_tune()->gemm_v2 == 2) && g->sa == LD_GEN && g->sb == LD_GEN && !g->tb && !g->c_upper && g->zero_rows == 0 &&
-         !ld_tune()->gemm_lowp && ld_tune()->gemm_v2 && ((uintptr_t)g->A & 15) == 0 &&
-         ((uintptr_t)g->B & 15) == 0 && (g->lda & 1) == 0 && (g->ldb & 1) == 0;
+  // (products with op(A) = A^T run on it only with gemm_v2 = 2: in the
+  // factorization they measured no faster than on the tiled kernel)
+  return (!g->ta || ld_tune()->gemm_v2 >= 2) && g->sa == LD_GEN && g->sb == LD_GEN && !g->tb &&
+         !g->c_upper && g->zero_rows == 0 && !ld_tune()->gemm_lowp && ld_tune()->gemm_v2 &&
+         ((uintptr_t)g->A & 15) == 0 && ((uintptr_t)g->B & 15) == 0 && (g->lda & 1) == 0 &&
+         (g->ldb & 1) == 0;
 }
 
 static void gemm2_general(const ld_gemm_t *g) {
@@ -627,20 +631,53 @@ static void gemm2_general(const ld_gemm_t *g) {
     attr = 1;
   }
   {
+    // split K when the tiles give fewer workgroups than the split-K target
+    // (512 by default: two per CU, the kernel's occupancy), down to 256 k
+    // per split: the partials are summed and scaled by the split-K reduction
+    int splits = 1, kchunk = Ki;
+    const long target = ld_tune()->gemm_min_wgs;
+    if (tiles < target && Ki >= 512) {
+      splits = (int)((target + tiles - 1) / tiles);
+      if (splits > Ki / 256) splits = Ki / 256;
+      const long cap = (64L << 20) / ((long)Mi * Ni);
+      if (splits > cap) splits = (int)cap;
+      if (splits < 1) splits = 1;
+      kchunk = ((Ki + splits - 1) / splits + G2_BK - 1) / G2_BK * G2_BK;
+      splits = (Ki + kchunk - 1) / kchunk;
+    }
+    double *P = nullptr;
+    size_t mark = 0;
+    if (splits > 1) {
+      mark = ld_ws_mark();
+      P = (double *)ld_ws_alloc((size_t)splits * Mi * Ni * sizeof(double));
+      if (ld_failed()) {
+        ld_ws_release(mark);
+        return;
+      }
+    }
     const int ph = ld_prof_begin(g->ta ? "gemm2_f64_kernel<true>" : "gemm2_f64_kernel<false>");
-    dim3 grid(Mi / G2_BM, Ni / G2_BN, 1);
+    dim3 grid(Mi / G2_BM, Ni / G2_BN, splits);
     if (g->ta)
       hipLaunchKernelGGL(gemm2_f64_kernel<true>, grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha, g->beta,
-                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, Ki, (double *)nullptr);
+                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P);
     else
       hipLaunchKernelGGL(gemm2_f64_kernel<false>, grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha, g->beta,
-                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, Ki, (double *)nullptr);
-    ld_prof_end(ph, 2.0 * Mi * Ni * Ki, 8.0 * ((double)Mi * Ki + (double)Ki * Ni + 2.0 * Mi * Ni));
+                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P);
+    ld_prof_end(ph, 2.0 * Mi * Ni * Ki,
+                8.0 * ((double)Mi * Ki + (double)Ki * Ni + (splits > 1 ? (double)splits : 2.0) * Mi * Ni));
+    if (splits > 1) {
+      const size_t tot = (size_t)Mi * Ni;
+      const int ph2 = ld_prof_begin("splitk_reduce_kernel");
+      hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((tot + 63) / 64)), dim3(256), 0, st, Mi, Ni,
+                         splits, g->alpha, g->beta, (const double *)P, g->C, g->ldc, 0, 0);
+      ld_prof_end(ph2, (double)splits * tot, 8.0 * ((double)splits * tot + 2.0 * tot));
+      ld_ws_release(mark);
+    }
   }
-  // the K tail onto the interior (beta = 1)
+  // the K tail onto the interior (beta = 1 for every row)
   if (Ki < K) {
     ld_gemm_t h = *g;
-    h.m = Mi; h.n = Ni; h.k = K - Ki; h.beta = 1.0;
+    h.m = Mi; h.n = Ni; h.k = K - Ki; h.beta = 1.0; h.zero_rows = 0;
     h.A = g->ta ? g->A + Ki : g->A + (size_t)Ki * g->lda;
     h.B = g->B + Ki;
     gemm_tiles(&h);
