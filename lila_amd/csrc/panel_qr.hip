@@ -18,11 +18,12 @@
 //
 // 2. panel_cqr_kernel -- communication-avoiding: Cholesky-QR twice
 //    (Gram X^T X of each slab on v_mfma_f64_16x16x4, one grid reduction per
-//    pass, the 32 x 32 Cholesky in registers of one wave, Q = X R^-1 by
-//    row-wise substitution with the rows in registers), then Householder
+//    pass, the 32 x 32 Cholesky in registers of one wave, Q = X R^-1 as an
+//    MFMA product with the explicit inverse), then Householder
 //    reconstruction: sign-chosen LU of the top nb x nb block of Q
-//    (Q1 S - I = L U), V1 = L, V2 = Q2 S U^-1, T = -U V1^-T, R = S R2 R1,
-//    explicit Q = Q S.  Three grid reductions per panel instead of nb + 1.  In exact arithmetic this is the same V, R, T
+//    (Q1 S - I = L U, formed by every workgroup from its own copy of the
+//    top rows), V1 = L, V2 = Q2 S U^-1, T = -U V1^-T, R = S R2 R1,
+//    explicit Q = Q S.  Two grid reductions per panel instead of nb + 1.  In exact arithmetic this is the same V, R, T
 //    and Q as the column-by-column Householder panel (Q and R are unique up
 //    to column signs and the reconstruction picks dlarfg's signs).  If a
 //    Cholesky pivot is not safely positive, the diagonal of R spreads by
@@ -626,9 +627,17 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   double *D0 = Sv + PNB;        // 1 / diag of the current triangular factor
   double *D2 = D0 + PNB;
   int *flag = (int *)(D2 + PNB);  // [0] proceed, [1] the LU's verdict
-  double *G1 = a.scr + OFF_G1, *G2 = a.scr + OFF_G2, *pub = a.scr + OFF_PUB;
+  double *G1 = a.scr + OFF_G1, *G2 = a.scr + OFF_G2;
   unsigned nbar = 0;
 
+  // the panel's top nb rows (X_top, column-major in M3 with stride LDM) go
+  // through both passes beside the slab, with the same arithmetic as those
+  // rows get in workgroup 0's slab, so that every workgroup forms the same
+  // Q_top -- and the same sign-chosen LU -- bit for bit, with no exchange
+  for (int t = tid; t < nb * nb; t += PNT) {
+    const int r = t % nb, c = t / nb;
+    M3[r + c * LDM] = (r < a.ml) ? a.A[(size_t)r + (size_t)c * a.lda] : 0.0;
+  }
   load_slab(a, X, r0, nloc);
 
   // ---- pass 1: G1 = X^T X, R1 = chol(G1), X <- X R1^-1 ----
@@ -655,7 +664,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   __syncthreads();
   if (flag[0]) {
     slab_mul_inv(X, RS, nloc, nb, M0, D0, nullptr, M4);
-    __syncthreads();
+    slab_mul(M3, LDM, nb, nb, nullptr, M4);  // X_top R1^-1
     // ---- pass 2 ----
     slab_gram(X, RS, nloc, nb, M2);
     __syncthreads();
@@ -681,32 +690,9 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   if (flag[0]) {
     slab_mul_inv(X, RS, nloc, nb, M2, D2, nullptr, M4);  // X = Q
-    __syncthreads();
+    slab_mul(M3, LDM, nb, nb, nullptr, M4);              // X_top = Q_top
     if (g == 0) {
-      // top nb x nb of Q into M3 (identity beyond nb)
-      for (int t = tid; t < PNB * PNB; t += PNT) {
-        const int r = t / PNB, c = t % PNB;
-        M3[r * LDM + c] = (r < nb && c < nb) ? X[r + c * RS] : (r == c ? 1.0 : 0.0);
-      }
-      __syncthreads();
-      if (wave == 0) {
-        const bool ok = wave_lu_sign32(M3, Sv, D2, nb);
-        if (tid == 0) flag[1] = ok ? 1 : 0;
-      }
-      __syncthreads();
-      // publish U (upper part of M3, identity beyond nb), S and the flag
-      for (int t = tid; t < PNB * PNB; t += PNT) {
-        const int r = t / PNB, c = t % PNB;
-        st_coh(&pub[r * PNB + c], r <= c ? M3[r * LDM + c] : 0.0);
-      }
-      if (tid < nb) st_coh(&pub[PNB * PNB + tid], Sv[tid]);
-      if (tid == 0) st_coh(&pub[PNB * PNB + PNB], flag[1] ? 1.0 : 0.0);
-    }
-    nbar++;
-    grid_arrive(a.cnt);
-    if (g == 0) {
-      // while the others wait: R = R2 R1 (both upper) on MFMA, one 16 x 16
-      // tile per wave, into M0
+      // R = R2 R1 (both upper) on MFMA, one 16 x 16 tile per wave, into M0
       const int ti = wave & 1, tj = wave >> 1;
       d4_t racc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -719,8 +705,18 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
       for (int v = 0; v < 4; v++) M0[(ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15)] = racc[v];
     }
-    grid_wait(a.cnt, nbar * (unsigned)G, a.err);
-    if (tid == 0) flag[0] = ld_coh(&pub[PNB * PNB + PNB]) != 0.0 ? 1 : 0;
+    __syncthreads();
+    // Q_top into M2 (row-major, identity beyond nb) and its sign-chosen LU
+    // in every workgroup: U on and above the diagonal, L below, S, 1 / pivots
+    for (int t = tid; t < PNB * PNB; t += PNT) {
+      const int r = t / PNB, c = t % PNB;
+      M2[r * LDM + c] = (r < nb && c < nb) ? M3[r + c * LDM] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const bool ok = wave_lu_sign32(M2, Sv, D2, nb);
+      if (tid == 0) flag[0] = ok ? 1 : 0;
+    }
     __syncthreads();
   }
 
@@ -732,14 +728,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     return;
   }
 
-  // ---- outputs: S, U (M2) and 1 / diag(U) in every workgroup ----
-  if (tid < nb) Sv[tid] = ld_coh(&pub[PNB * PNB + tid]);
-  for (int t = tid; t < PNB * PNB; t += PNT) {
-    const int r = t / PNB, c = t % PNB;
-    M2[r * LDM + c] = ld_coh(&pub[r * PNB + c]);
-  }
-  __syncthreads();
-  if (tid < PNB) D2[tid] = 1.0 / M2[tid * LDM + tid];
+  // ---- outputs ----
   // explicit panel Q = Q S
   if (a.Q) store_slab(X, RS, nb, nloc, a.Q + r0, a.ldq, Sv);
   __syncthreads();
@@ -750,28 +739,31 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     // top nb rows: R' = S R2 R1 on and above the diagonal, V1 = L below
     for (int t = tid; t < nb * nb; t += PNT) {
       const int r = t % nb, c = t / nb;
-      X[r + c * RS] = (r <= c) ? Sv[r] * M0[r * LDM + c] : M3[r * LDM + c];
+      X[r + c * RS] = (r <= c) ? Sv[r] * M0[r * LDM + c] : M2[r * LDM + c];
     }
-    // T = -U V1^-T, one row of T per thread: T V1^T = -U, so with t = row
-    // r of T, right-looking over p: t_c -= t_p V1(c,p) for c > p
-    if (tid < nb) {
-      const int r = tid;
-      double t[PNB];
+    // T = -U V1^-T = -U (V1^T)^-1: V1^T (unit upper, identity beyond nb)
+    // into M3, its inverse into M4 by one wave, then the product of the two
+    // upper triangles on MFMA
+    for (int t = tid; t < PNB * PNB; t += PNT) {
+      const int r = t / PNB, c = t % PNB;
+      M3[r * LDM + c] = (r < c) ? M2[c * LDM + r] : (r == c ? 1.0 : 0.0);
+    }
+    if (tid < PNB) D0[tid] = 1.0;
+    __syncthreads();
+    if (tid < 64) small32::inv_upper32(M3, D0, M4);
+    __syncthreads();
+    const int ti = wave & 1, tj = wave >> 1;
+    d4_t tacc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int c = 0; c < PNB; c++) {
-        const double u = -M2[r * LDM + c];  // 0 beyond nb (U is padded)
-        t[c] = (c >= r) ? u : 0.0;
-      }
+    for (int ks = 0; ks < PNB / 4; ks++) {
+      const int k = 4 * ks + (lane >> 4), i = ti * 16 + (lane & 15);
+      const double u = (i <= k) ? M2[i * LDM + k] : 0.0;  // U; L lies below
+      tacc = __builtin_amdgcn_mfma_f64_16x16x4f64(u, M4[k * LDM + tj * 16 + (lane & 15)], tacc, 0, 0, 0);
+    }
 #pragma unroll
-      for (int p = 0; p < PNB; p++) {
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int c = p + 1; c < PNB; c++) t[c] -= t[p] * M3[c * LDM + p];
-      }
-      // t_c = 0 for c >= nb; as above, column nb - 1 is written last
-#pragma unroll
-      for (int c = PNB - 1; c >= 0; c--)
-        if (c >= r) a.T[(size_t)r + (size_t)min(c, nb - 1) * a.ldt] = t[c];
+    for (int v = 0; v < 4; v++) {
+      const int r = ti * 16 + (lane >> 4) + 4 * v, c = tj * 16 + (lane & 15);
+      if (r <= c && c < nb) a.T[(size_t)r + (size_t)c * a.ldt] = -tacc[v];
     }
   }
   __syncthreads();
