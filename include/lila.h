@@ -415,8 +415,11 @@ int lila_autotune_form(int form);
  * column-by-column kernel as in-kernel fallback; 0 always column-by-column
  * (dgeqr3's arithmetic).  Both give dgeqr3's V, R, T and Q to rounding. */
 int lila_set_panel_mode(int mode);
-/* Panel width nb (8..32, default 32): dgeqr3's recursion hands widths up to
- * nb to one panel launch.  Returns the previous width, -1 out of range. */
+/* Panel width nb (8..32 or 64, default 64): dgeqr3's recursion hands widths
+ * up to nb to one panel launch; with 64, widths 33..64 go to the 64-column
+ * Cholesky-QR2 panel where its LDS slabs hold the panel (up to ~20700 rows;
+ * taller ones recurse down to 32-wide panels).  Returns the previous width,
+ * -1 out of range. */
 int lila_set_panel_width(int nb);
 /* VRTQ recursion (lila_dgeqrf_w03_recursive with hh or LAPACK leaves): 0
  * (default) builds Q at every node as the reference's call tree does, 1

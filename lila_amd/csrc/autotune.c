@@ -177,9 +177,9 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
     }
     {
         /* panel width nb (dgeqr3's recursion cutoff) and the panel algorithm */
-        if (tu->panel_nb < 8 || tu->panel_nb > 32) tu->panel_nb = 32;
-        static const int v[] = {32, 16, 24};
-        sweep(&c, "panel_nb", &tu->panel_nb, v, 3);
+        if ((tu->panel_nb < 8 || tu->panel_nb > 32) && tu->panel_nb != 64) tu->panel_nb = 32;
+        static const int v[] = {32, 64, 16, 24};
+        sweep(&c, "panel_nb", &tu->panel_nb, v, 4);
         static const int pm[] = {1, 0};
         sweep(&c, "panel_mode", &tu->panel_mode, pm, 2);
         /* Q built at every node (the reference's call tree) or after the VRT

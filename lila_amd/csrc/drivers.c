@@ -256,6 +256,9 @@ static int panel_width(void) {
 
 int dv_dgeqr3(int m, int n, double *A, int lda, double *T, int ldt) {
     if (n <= 0) return 0;
+    if (ld_tune()->panel_nb == 64 && n > LD_PANEL_MAX && n <= 64 &&
+        ld_panel_qr64(m, n, A, lda, T, ldt) == 0)
+        return 0;
     if (n <= panel_width()) {
         panel(m, n, A, lda, T, ldt, NULL, 0);
         return 0;

@@ -69,6 +69,9 @@ void ld_sub(int m, int n, const double *W, int ldw, double *C, int ldc); /* C -=
 #define LD_PANEL_MAX 32
 void ld_panel_qr(int ml, int n, double *A, int lda, double *T, int ldt,
                  double *Q, int ldq);
+/* the 64-wide Cholesky-QR2 panel (32 < n <= 64, no explicit Q, panel mode
+ * 1): 0 launched, -1 not taken (nothing launched) */
+int ld_panel_qr64(int ml, int n, double *A, int lda, double *T, int ldt);
 /* tall panels (taller than the workgroups' LDS slabs hold): */
 int ld_panel_is_tall(int ml, int n);
 /* the streamed Cholesky-QR2 panel without its column-loop fallback: 0 done,
@@ -200,7 +203,9 @@ typedef struct {
     int gemm_small_tiles_t;  /* gemm_small_tiles for products with op(A) = A^T */
     int batch_grid;          /* batched least squares: workgroups of the persistent launch
                                 (0: one per CU at the kernel's occupancy) */
-    int panel_nb;            /* panel width: dgeqr3 recursion cutoff (8..32; 0 = 32) */
+    int panel_nb;            /* panel width: dgeqr3 recursion cutoff (8..32, or 64: widths
+                                33..64 to the 64-wide Cholesky-QR2 panel where it fits;
+                                0 = 32) */
     int tune_form;           /* lila_autotune times the plain (0) or mixed (1) form */
     int gemm_kmin;           /* split-K: rows of K per split at least (0 = 64) */
     int gemm_v2;             /* 1: general products on the 128 x 128 direct-to-LDS kernel */

@@ -175,11 +175,14 @@ __device__ __forceinline__ size_t slab_doubles(int nb, int rs) { return ((size_t
 
 // rows [r0, r0 + nloc) of an nb-column block of a column-major matrix into
 // the slab and back: one row per thread, eight columns in flight
+// the slab's nloc x nb entries, every thread taking entries nloc apart
+// (loads coalesced over the rows, all of a thread's loads independent)
 __device__ void load_slab(const PanelArgs &a, double *slab, int r0, int nloc) {
-  for (int lr = threadIdx.x; lr < nloc; lr += PNT) {
-    const double *src = a.A + (size_t)(r0 + lr);
-#pragma unroll 8
-    for (int c = 0; c < a.nb; c++) slab[lr + c * a.rs] = src[(size_t)c * a.lda];
+  const int tot = nloc * a.nb;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < tot; e += PNT) {
+    const int lr = e % nloc, c = e / nloc;
+    slab[lr + c * a.rs] = a.A[(size_t)(r0 + lr) + (size_t)c * a.lda];
   }
   __syncthreads();
 }
@@ -772,6 +775,576 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 
 // ---------------------------------------------------------------------------
+// 64-column Cholesky-QR2 + Householder reconstruction: the algorithm of
+// panel_cqr_kernel on an ml x nb panel with 32 < nb <= 64 (dgeqr3's two
+// 32-column panels, the update between them and the T join in one launch).
+// The 64 x 64 triangular factors are 2 x 2 grids of 32 x 32 blocks (ld LDM)
+// handled by the one-wave 32 x 32 routines of small32.h and 32 x 32 x 32
+// MFMA products by all four waves:
+//   chol64: R11 = chol(A11), R12 = R11^-T A12, R22 = chol(A22 - R12^T R12);
+//   inverse: [X11 X12; 0 X22], X12 = -X11 R12 X22;
+//   sign-chosen LU of Q_top (B = Q S - I, the column rule decided on the
+//   fly): LU of Q11 (signs S1), L21 = (Q21 S1) U11^-1, Y = L11^-1 Q12, LU of
+//   Q22 - L21 Y (signs S2), U12 = Y S2.
+// Upper-triangular 64 x 64 matrices are stored as three blocks (11, 12, 22),
+// the LU as four.  A panel that fails an acceptance test is factored by the
+// column-by-column loop in the same launch: its left 32 columns, their block
+// reflector applied to the right ones (one grid reduction of V1^T A2), the
+// right columns below row 32, and the T join (a grid reduction of V1^T V2).
+// ---------------------------------------------------------------------------
+constexpr int B32 = PNB * LDM;                  // doubles of one 32 x 32 block
+constexpr int OFF64_G1 = SCRATCH;               // 64 x 64 Gram, pass 1
+constexpr int OFF64_G2 = OFF64_G1 + 4096;       // pass 2
+constexpr int OFF64_W = OFF64_G2 + 4096;        // fallback: V1^T A2 (32 x 32)
+constexpr int OFF64_H = OFF64_W + 1024;         // fallback: V1^T V2 (32 x 32)
+constexpr int SCRATCH64 = OFF64_H + 1024;
+// LDS past the slab: M0 (3 blocks), M2 (4), M4 (3), S, D0, D2, ones (64
+// each), flags
+constexpr int CQR64_TAIL = 10 * B32 + 4 * 64 + 4;
+static_assert(HH_TAIL <= 7 * B32, "the column loop's LDS inside M2 and M4");
+
+// block (bi, bj), bi <= bj, of an upper 64 x 64 matrix stored as 3 blocks
+__device__ __forceinline__ double *ublk(double *M, int bi, int bj) { return M + (bi + bj) * B32; }
+// block (bi, bj) of a full 64 x 64 matrix stored as 4 blocks
+__device__ __forceinline__ double *fblk(double *M, int bi, int bj) { return M + (2 * bi + bj) * B32; }
+
+// entry (r, c) of an upper 64 x 64 matrix in 3-block storage (0 below)
+__device__ __forceinline__ double uval(const double *M, int r, int c) {
+  if (r >= 32 && c < 32) return 0.0;
+  return M[((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)];
+}
+// U(r, c) of the LU in 4-block storage: its upper part (L lies below)
+__device__ __forceinline__ double luu(const double *M, int r, int c) {
+  if (r > c) return 0.0;
+  return M[(2 * (r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)];
+}
+// L(r, c), r > c, of the LU in 4-block storage (strict lower part)
+__device__ __forceinline__ double lul(const double *M, int r, int c) {
+  return M[(2 * (r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)];
+}
+
+// C = alpha A B (+ C when acc) for 32 x 32 blocks, all four waves (wave w
+// the 16 x 16 output tile (w & 1, w >> 1)): A(i, k) at A[i ars + k acs] (0
+// for i > k when amask: an upper factor whose lower part holds something
+// else), B(k, j) at B[k brs + j bcs], C row-major with ld LDM.  C may alias
+// A or B: every read precedes the first write.
+__device__ void wg_mm32(const double *A, int ars, int acs, bool amask, const double *B, int brs, int bcs,
+                        double *C, double alpha, bool acc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ti = wave & 1, tj = wave >> 1, i = ti * 16 + (lane & 15), j = tj * 16 + (lane & 15);
+  d4_t r = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < 8; ks++) {
+    const int k = 4 * ks + (lane >> 4);
+    const double av = (amask && i > k) ? 0.0 : A[i * ars + k * acs];
+    r = __builtin_amdgcn_mfma_f64_16x16x4f64(av, B[k * brs + j * bcs], r, 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double *c = C + (ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15);
+    *c = acc ? *c + alpha * r[v] : alpha * r[v];
+  }
+  __syncthreads();
+}
+
+// upper Cholesky of the 64 x 64 G in M (3-block storage, upper part read,
+// identity beyond nb) in place; 1 / diag(R) into D; Sc a free block.
+// Returns false when a pivot is not safely positive or R's diagonal over
+// the first nb columns spreads by more than 1e5.
+__device__ bool wg_chol64(double *M, double *D, double *Sc, int nb, int *flag) {
+  const int tid = threadIdx.x, wave = tid >> 6;
+  if (wave == 0) {
+    const bool ok = wave_chol32(ublk(M, 0, 0), D, nb < 32 ? nb : 32);
+    if (tid == 0) flag[0] = ok ? 1 : 0;
+  }
+  __syncthreads();
+  if (wave == 0) small32::inv_upper32(ublk(M, 0, 0), D, Sc);  // R11^-1
+  __syncthreads();
+  // R12 = R11^-T A12, then A22 -= R12^T R12
+  wg_mm32(Sc, 1, LDM, false, ublk(M, 0, 1), LDM, 1, ublk(M, 0, 1), 1.0, false);
+  wg_mm32(ublk(M, 0, 1), 1, LDM, false, ublk(M, 0, 1), LDM, 1, ublk(M, 1, 1), -1.0, true);
+  if (wave == 0) {
+    const bool ok = wave_chol32(ublk(M, 1, 1), D + 32, nb > 32 ? nb - 32 : 0);
+    if (tid == 0) flag[0] = (flag[0] && ok) ? 1 : 0;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    // R's diagonal spread over the first nb columns: max/min of diag(R) is
+    // max/min of D = 1 / diag(R)
+    const int lane = tid & 63;
+    double dmn = (lane < nb) ? D[lane] : 1e308, dmx = (lane < nb) ? D[lane] : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      dmn = fmin(dmn, __shfl_xor(dmn, o));
+      dmx = fmax(dmx, __shfl_xor(dmx, o));
+    }
+    if (lane == 0 && !(dmn > 1e-5 * dmx)) flag[0] = 0;
+  }
+  __syncthreads();
+  return flag[0] != 0;
+}
+
+// X = R^-1 for the upper 64 x 64 R given as its blocks R11, R12, R22 (lower
+// parts of R11, R22 not read), D = 1 / diag(R); X in 3-block storage
+__device__ void wg_inv64(const double *R11, const double *R12, const double *R22, const double *D,
+                         double *X) {
+  const int wave = threadIdx.x >> 6;
+  if (wave == 0) small32::inv_upper32(R11, D, ublk(X, 0, 0));
+  else if (wave == 1) small32::inv_upper32(R22, D + 32, ublk(X, 1, 1));
+  __syncthreads();
+  // X12 = -(X11 R12) X22
+  wg_mm32(ublk(X, 0, 0), LDM, 1, false, R12, LDM, 1, ublk(X, 0, 1), 1.0, false);
+  wg_mm32(ublk(X, 0, 1), LDM, 1, false, ublk(X, 1, 1), LDM, 1, ublk(X, 0, 1), -1.0, false);
+}
+
+// X (nrows rows, stride RS, the first nb of 64 columns) <- (X diag(s)) MI
+// for the upper 64 x 64 MI in 3-block storage (s == nullptr: no scaling):
+// 16-row tiles on MFMA, a wave reading all of its tile's rows before
+// writing them; each output row depends on its own input row only
+__device__ void slab_mul64(double *X, int RS, int nrows, int nb, const double *s, double *MI) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cl = lane & 15;
+  const int ntile = (nrows + 15) / 16;
+  // this lane's MFMA operands of MI (the same for every tile), diag(s)
+  // folded in: (X diag(s)) MI = X (diag(s) MI)
+  double mf[16][4];
+#pragma unroll
+  for (int ks = 0; ks < 16; ks++) {
+    const int k = 4 * ks + (lane >> 4);
+    const double sk = (s && k < nb) ? s[k] : 1.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) mf[ks][q] = (4 * ks <= 16 * q + 15) ? sk * uval(MI, k, 16 * q + cl) : 0.0;
+  }
+  for (int t = wave; t < ntile; t += PNT / 64) {
+    const int row = 16 * t + cl;
+    d4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < 16; ks++) {
+      const int k = 4 * ks + (lane >> 4);
+      const double a = (row < nrows && k < nb) ? X[row + k * RS] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (4 * ks <= 16 * q + 15)  // MI upper: rows k <= 16 q + 15 only
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, mf[ks][q], acc[q], 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = 16 * t + (lane >> 4) + 4 * v;
+      if (r < nrows)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (16 * q + cl < nb) X[r + (16 * q + cl) * RS] = acc[q][v];
+    }
+  }
+  __syncthreads();
+}
+
+// Gram X^T X of the slab's first nloc rows (64 columns, upper 16 x 16 tiles
+// dealt to the waves), its upper part added into the global G (64 x 64)
+__device__ void slab_gram64_add(const double *X, int RS, int nloc, int nb, double *G) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int u = wave; u < 10; u += 4) {
+    // tile u of (0,0) (0,1) (0,2) (0,3) (1,1) (1,2) (1,3) (2,2) (2,3) (3,3)
+    const int ti = (u < 4) ? 0 : (u < 7) ? 1 : (u < 9) ? 2 : 3;
+    const int tj = (u < 4) ? u : (u < 7) ? u - 3 : (u < 9) ? u - 5 : 3;
+    const int ci = ti * 16 + (lane & 15), cj = tj * 16 + (lane & 15);
+    const bool vi = ci < nb, vj = cj < nb;
+    d4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = d4_t{0.0, 0.0, 0.0, 0.0};
+    int k0 = 0;
+    for (; k0 + 16 <= nloc; k0 += 16) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int k = k0 + 4 * q + (lane >> 4);
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(vi ? X[k + ci * RS] : 0.0, vj ? X[k + cj * RS] : 0.0,
+                                                      acc[q], 0, 0, 0);
+      }
+    }
+    for (; k0 < nloc; k0 += 4) {
+      const int k = k0 + (lane >> 4);
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64((vi && k < nloc) ? X[k + ci * RS] : 0.0,
+                                                    (vj && k < nloc) ? X[k + cj * RS] : 0.0, acc[0], 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = ti * 16 + (lane >> 4) + 4 * v, c = tj * 16 + (lane & 15);
+      const double d = (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
+      if (r <= c && c < nb) add_coh(&G[r * 64 + c], d);
+    }
+  }
+}
+
+// the reduced Gram (upper, first nb) into M (3-block storage), identity
+// beyond nb, zero below the diagonal
+__device__ void gram64_read(const double *G, int nb, double *M) {
+  for (int t = threadIdx.x; t < 64 * 64; t += PNT) {
+    const int r = t >> 6, c = t & 63;
+    if (r >= 32 && c < 32) continue;
+    double v = 0.0;
+    if (r < nb && c < nb) { if (r <= c) v = ld_coh(&G[r * 64 + c]); }
+    else if (r == c) v = 1.0;
+    M[((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)] = v;
+  }
+  __syncthreads();
+}
+
+// sign-chosen LU of Q_top (4-block storage in M, identity beyond nb) in
+// place: U on and above the diagonal, L below; S the signs, D 1 / pivots.
+// Sc: 3 free blocks, one1: 32 ones.  Returns false when some |t| > 1 -
+// 1e-10 (tau within 1e-10 of 0 or 2).
+__device__ bool wg_lu64(double *M, double *S, double *D, double *Sc, const double *one1, int nb, int *flag) {
+  const int tid = threadIdx.x, wave = tid >> 6;
+  double *Q11 = fblk(M, 0, 0), *Q12 = fblk(M, 0, 1), *Q21 = fblk(M, 1, 0), *Q22 = fblk(M, 1, 1);
+  double *Ui = Sc, *Lt = Sc + B32, *Lti = Sc + 2 * B32;
+  if (wave == 0) {
+    const bool ok = wave_lu_sign32(Q11, S, D, nb < 32 ? nb : 32);
+    if (tid == 0) flag[0] = ok ? 1 : 0;
+  }
+  __syncthreads();
+  // U11^-1 (wave 0) and (L11^T)^-1 = (L11^-1)^T (wave 1); Q21 S1
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t / PNB, c = t % PNB;
+    Lt[r * LDM + c] = (r < c) ? Q11[c * LDM + r] : (r == c ? 1.0 : 0.0);
+    Q21[r * LDM + c] *= S[c];
+  }
+  __syncthreads();
+  if (wave == 0) small32::inv_upper32(Q11, D, Ui);
+  else if (wave == 1) small32::inv_upper32(Lt, one1, Lti);
+  __syncthreads();
+  wg_mm32(Q21, LDM, 1, false, Ui, LDM, 1, Q21, 1.0, false);    // L21 = (Q21 S1) U11^-1
+  wg_mm32(Lti, 1, LDM, false, Q12, LDM, 1, Q12, 1.0, false);   // Y = L11^-1 Q12
+  wg_mm32(Q21, LDM, 1, false, Q12, LDM, 1, Q22, -1.0, true);   // Q22 - L21 Y
+  if (wave == 0) {
+    const bool ok = wave_lu_sign32(Q22, S + 32, D + 32, nb > 32 ? nb - 32 : 0);
+    if (tid == 0) flag[0] = (flag[0] && ok) ? 1 : 0;
+  }
+  __syncthreads();
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t / PNB, c = t % PNB;
+    Q12[r * LDM + c] *= S[32 + c];  // U12 = Y S2
+  }
+  __syncthreads();
+  return flag[0] != 0;
+}
+
+// last workgroup out re-zeroes the 64-wide accumulation buffers and the
+// counters
+__device__ void exit_protocol64(const PanelArgs &a) {
+  __syncthreads();
+  __shared__ unsigned last64;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    const unsigned old = __hip_atomic_fetch_add(&a.cnt[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    last64 = (old == (unsigned)a.G - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (last64) {
+    for (int t = tid; t < 3 * SLOT; t += PNT) st_coh(&a.scr[OFF_RED + t], 0.0);
+    for (int t = tid; t < SCRATCH64 - OFF64_G1; t += PNT) st_coh(&a.scr[OFF64_G1 + t], 0.0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) {
+      __hip_atomic_store(&a.cnt[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&a.cnt[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// V1(r0 + lr, k) of the slab after the left columns' column loop: unit
+// lower in the panel's rows 0..31 (R11 lies above), as stored below
+__device__ __forceinline__ double v1val(const double *slab, int RS, int r0, int lr, int k) {
+  const int r = r0 + lr;
+  return (r < k) ? 0.0 : (r == k ? 1.0 : slab[lr + k * RS]);
+}
+
+// the column-by-column fallback of the 64-wide panel (slab reloaded, A, T
+// untouched so far): left 32 columns, their reflectors on the right ones,
+// the right columns below row 32, the T join.  tail: the column loop's LDS
+// (HH_TAIL doubles); blk: three free 32 x 32 blocks (T1, W, H).
+__device__ __noinline__ void hh_fallback64(const PanelArgs &a, int tail_off, int blk_off, int r0, int nloc,
+                                           unsigned &nbar) {
+  // (the LDS pointers are formed here from the dynamic LDS array: passed in
+  // as generic pointers and offset, the compiler's null checks of their
+  // casts produce an instruction it cannot encode)
+  extern __shared__ double smem[];
+  double *slab = smem, *tail = smem + tail_off, *blk = smem + blk_off;
+  double *T1 = blk, *Wl = blk + B32, *Hl = blk + 2 * B32;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = blockIdx.x;
+  const int RS = a.rs, G = a.G, nb2 = a.nb - 32;
+  PanelArgs a1 = a;
+  a1.nb = 32;
+  a1.Q = nullptr;
+  hh_factor<false>(a1, slab, tail, r0, nloc, nbar);
+  // every workgroup holds T1 (the column loop forms it from reduced values)
+  const double *Tl = tail + PNB * PNB;
+  for (int t = tid; t < PNB * PNB; t += PNT) T1[(t / PNB) * LDM + t % PNB] = Tl[(t / PNB) + PNB * (t % PNB)];
+  __syncthreads();
+  // ---- W = V1^T A2 over all rows (grid reduction), W' = T1^T W ----
+  double *W = a.scr + OFF64_W;
+  {
+    const int ti = wave & 1, tj = wave >> 1;
+    const int ci = ti * 16 + (lane & 15), cj = tj * 16 + (lane & 15);
+    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nloc; k0 += 4) {
+      const int lr = k0 + (lane >> 4);
+      const double x = (lr < nloc) ? v1val(slab, RS, r0, lr, ci) : 0.0;
+      const double y = (lr < nloc && cj < nb2) ? slab[lr + (32 + cj) * RS] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = ti * 16 + (lane >> 4) + 4 * v, c = tj * 16 + (lane & 15);
+      if (acc[v] != 0.0) add_coh(&W[r * PNB + c], acc[v]);
+    }
+  }
+  nbar++;
+  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  for (int t = tid; t < PNB * PNB; t += PNT) Wl[(t / PNB) * LDM + t % PNB] = ld_coh(&W[t]);
+  __syncthreads();
+  // W' = T1^T W (T1 upper), into Wl
+  wg_mm32(T1, 1, LDM, false, Wl, LDM, 1, Wl, 1.0, false);
+  // A2 -= V1 W' on this slab's rows
+  for (int t = wave; t < (nloc + 15) / 16; t += PNT / 64) {
+    const int lr = 16 * t + (lane & 15);
+    d4_t acc[2] = {d4_t{0.0, 0.0, 0.0, 0.0}, d4_t{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) {
+      const int k = 4 * ks + (lane >> 4);
+      const double x = (lr < nloc) ? v1val(slab, RS, r0, lr, k) : 0.0;
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, Wl[k * LDM + (lane & 15)], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, Wl[k * LDM + 16 + (lane & 15)], acc[1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = 16 * t + (lane >> 4) + 4 * v;
+      if (r < nloc) {
+        if ((lane & 15) < nb2) slab[r + (32 + (lane & 15)) * RS] -= acc[0][v];
+        if (16 + (lane & 15) < nb2) slab[r + (48 + (lane & 15)) * RS] -= acc[1][v];
+      }
+    }
+  }
+  __syncthreads();
+  // R12: rows 0..31 of the right columns, in workgroup 0's slab
+  if (g == 0)
+    for (int t = tid; t < 32 * nb2; t += PNT) {
+      const int r = t % 32, c = t / 32;
+      a.A[(size_t)r + (size_t)(32 + c) * a.lda] = slab[r + (32 + c) * RS];
+    }
+  // ---- the right columns below row 32 by the column loop ----
+  // The right columns move to slab columns 0..31 (V1 to 32..63), and
+  // workgroup 0's rows up by 32 (its first 32 rows are done), so that the
+  // column loop runs on the slab as it stands.  (Handing it an offset slab
+  // pointer instead made the compiler emit an address-space check that it
+  // cannot encode.)
+  const int skip = (g == 0) ? 32 : 0;
+  const int nloc2 = nloc - skip, r02 = r0 + skip - 32;  // rows of the right sub-panel
+  // (a slab has fewer rows than the workgroup has threads: one row each)
+  for (int c0 = 0; c0 < 32; c0 += 4) {
+    double lo[4], hi[4];
+    const int lr = tid;
+    if (lr < nloc2) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        lo[q] = slab[lr + skip + (c0 + q) * RS];
+        hi[q] = slab[lr + skip + (32 + c0 + q) * RS];
+      }
+    }
+    __syncthreads();
+    if (lr < nloc2) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        slab[lr + (c0 + q) * RS] = hi[q];
+        slab[lr + (32 + c0 + q) * RS] = lo[q];
+      }
+    }
+    __syncthreads();
+  }
+  PanelArgs a2 = a;
+  a2.ml = a.ml - 32;
+  a2.nb = nb2;
+  a2.A = a.A + 32 + (size_t)32 * a.lda;
+  a2.T = a.T + 32 + (size_t)32 * a.ldt;
+  a2.Q = nullptr;
+  hh_factor<false>(a2, slab, tail, r02, nloc2, nbar);
+  // ---- T12 = -T1 (V1^T V2) T2 (V1 below row 32, V2 unit lower there) ----
+  double *H = a.scr + OFF64_H;
+  {
+    const int ti = wave & 1, tj = wave >> 1;
+    const int ci = ti * 16 + (lane & 15), cj = tj * 16 + (lane & 15);
+    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nloc2; k0 += 4) {
+      const int lr = k0 + (lane >> 4), rr = r02 + lr;  // row of the right sub-panel
+      double y = 0.0;
+      if (lr < nloc2 && cj < nb2) y = (rr < cj) ? 0.0 : (rr == cj ? 1.0 : slab[lr + cj * RS]);
+      const double x = (lr < nloc2) ? slab[lr + (32 + ci) * RS] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = ti * 16 + (lane >> 4) + 4 * v, c = tj * 16 + (lane & 15);
+      if (acc[v] != 0.0) add_coh(&H[r * PNB + c], acc[v]);
+    }
+  }
+  nbar++;
+  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  if (g == 0) {
+    // T2 (the column loop's T of the right columns, in LDS after it) and H
+    const double *T2 = tail + PNB * PNB;  // the column loop's T, T2[r + PNB c]
+    for (int t = tid; t < PNB * PNB; t += PNT) {
+      const int r = t / PNB, c = t % PNB;
+      Hl[r * LDM + c] = ld_coh(&H[t]);
+      Wl[r * LDM + c] = (c < nb2) ? T2[r + PNB * c] : 0.0;  // T2 row-major, zero beyond nb2
+    }
+    __syncthreads();
+    wg_mm32(T1, LDM, 1, false, Hl, LDM, 1, Hl, 1.0, false);    // T1 H
+    wg_mm32(Hl, LDM, 1, false, Wl, LDM, 1, Hl, -1.0, false);   // -(T1 H) T2
+    for (int t = tid; t < 32 * nb2; t += PNT) {
+      const int r = t % 32, c = t / 32;
+      a.T[(size_t)r + (size_t)(32 + c) * a.ldt] = Hl[r * LDM + c];
+    }
+  }
+}
+
+__global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) void panel_cqr64_kernel(PanelArgs a) {
+  extern __shared__ double smem[];
+  const int nb = a.nb, RS = a.rs, G = a.G;
+  const int g = blockIdx.x, tid = threadIdx.x;
+  const int r0 = g * a.rp;
+  const int nloc = max(0, min(a.ml, r0 + a.rp) - r0);
+  const int ntop = min(64, a.ml);  // rows of X_top
+  double *X = smem;                 // rp slab rows, then X_top at rows rp .. rp + 63
+  double *tail = smem + slab_doubles(64, RS);
+  double *M0 = tail;                // R1, then R (3 blocks)
+  double *M2 = M0 + 3 * B32;        // R2 (3-block storage), then the LU (4 blocks)
+  double *M4 = M2 + 4 * B32;        // inverses and scratch (3 blocks)
+  double *Sv = M4 + 3 * B32;        // 64 signs
+  double *D0 = Sv + 64;             // 1 / diag of the current factor
+  double *D2 = D0 + 64;
+  double *one = D2 + 64;
+  int *flag = (int *)(one + 64);
+  double *G1 = a.scr + OFF64_G1, *G2 = a.scr + OFF64_G2;
+  unsigned nbar = 0;
+
+  // slab and, at rows rp .., the panel's top 64 rows (see panel_cqr_kernel)
+  for (int t = tid; t < 64 * 64; t += PNT) {
+    const int r = t & 63, c = t >> 6;
+    if (c < nb) X[a.rp + r + c * RS] = (r < ntop) ? a.A[(size_t)r + (size_t)c * a.lda] : 0.0;
+  }
+  if (tid < 64) one[tid] = 1.0;
+  load_slab(a, X, r0, nloc);
+
+  // ---- pass 1 ----
+  slab_gram64_add(X, RS, nloc, nb, G1);
+  nbar++;
+  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  gram64_read(G1, nb, M0);
+  bool ok = wg_chol64(M0, D0, M4, nb, flag);
+  if (ok) {
+    wg_inv64(ublk(M0, 0, 0), ublk(M0, 0, 1), ublk(M0, 1, 1), D0, M4);
+    slab_mul64(X, RS, a.rp + 64, nb, nullptr, M4);
+    // ---- pass 2 ----
+    slab_gram64_add(X, RS, nloc, nb, G2);
+    nbar++;
+    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    gram64_read(G2, nb, M2);
+    ok = wg_chol64(M2, D2, M4, nb, flag);
+  }
+  if (ok) {
+    wg_inv64(ublk(M2, 0, 0), ublk(M2, 0, 1), ublk(M2, 1, 1), D2, M4);
+    slab_mul64(X, RS, a.rp + 64, nb, nullptr, M4);  // Q, and Q_top at rows rp ..
+    if (g == 0) {
+      // R = R2 R1 into M0: the three upper 32 x 32 blocks, each from
+      // registers after all reads
+      const int lane = tid & 63, wave = tid >> 6;
+      const int ti = wave & 1, tj = wave >> 1;
+      d4_t acc[3];
+#pragma unroll
+      for (int bq = 0; bq < 3; bq++) {
+        const int bi = (bq == 2) ? 1 : 0, bj = (bq == 0) ? 0 : 1;
+        acc[bq] = d4_t{0.0, 0.0, 0.0, 0.0};
+        const int i = 32 * bi + ti * 16 + (lane & 15), j = 32 * bj + tj * 16 + (lane & 15);
+        for (int ks = 8 * bi; ks < 8 * bj + 8; ks++) {
+          const int k = 4 * ks + (lane >> 4);
+          acc[bq] = __builtin_amdgcn_mfma_f64_16x16x4f64(uval(M2, i, k), uval(M0, k, j), acc[bq], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int bq = 0; bq < 3; bq++) {
+        const int bi = (bq == 2) ? 1 : 0, bj = (bq == 0) ? 0 : 1;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          ublk(M0, bi, bj)[(ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15)] = acc[bq][v];
+      }
+    }
+    __syncthreads();
+    // Q_top into M2 (4 blocks, identity beyond nb), its sign-chosen LU
+    for (int t = tid; t < 64 * 64; t += PNT) {
+      const int r = t >> 6, c = t & 63;
+      fblk(M2, r >> 5, c >> 5)[(r & 31) * LDM + (c & 31)] =
+          (r < nb && c < nb) ? X[a.rp + r + c * RS] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    ok = wg_lu64(M2, Sv, D2, M4, one, nb, flag);
+  }
+
+  if (!ok) {
+    load_slab(a, X, r0, nloc);
+    hh_fallback64(a, (int)(M2 - smem), (int)(M0 - smem), r0, nloc, nbar);
+    exit_protocol64(a);
+    return;
+  }
+
+  // ---- outputs: V2 = (Q2 S) U^-1 ----
+  wg_inv64(fblk(M2, 0, 0), fblk(M2, 0, 1), fblk(M2, 1, 1), D2, M4);
+  slab_mul64(X, RS, nloc, nb, Sv, M4);
+  if (g == 0) {
+    // top nb rows: R' = S R2 R1 on and above the diagonal, V1 = L below
+    for (int t = tid; t < nb * nb; t += PNT) {
+      const int r = t % nb, c = t / nb;
+      X[r + c * RS] = (r <= c) ? Sv[r] * uval(M0, r, c) : lul(M2, r, c);
+    }
+  }
+  if (g == G - 1) {
+    // (every workgroup holds U and L: the last one, with the fewest rows,
+    // forms T while workgroup 0 writes the top rows; with one workgroup,
+    // after them)
+    __syncthreads();
+    // T = -U (V1^T)^-1: V1^T (unit upper, identity beyond nb) into M0's
+    // blocks, its inverse into M4, then the product of the two triangles
+    for (int t = tid; t < 64 * 64; t += PNT) {
+      const int r = t >> 6, c = t & 63;
+      if (r >= 32 && c < 32) continue;
+      ublk(M0, r >> 5, c >> 5)[(r & 31) * LDM + (c & 31)] = (r < c) ? lul(M2, c, r) : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    wg_inv64(ublk(M0, 0, 0), ublk(M0, 0, 1), ublk(M0, 1, 1), one, M4);
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int u = wave; u < 10; u += 4) {
+      const int ti = (u < 4) ? 0 : (u < 7) ? 1 : (u < 9) ? 2 : 3;
+      const int tj = (u < 4) ? u : (u < 7) ? u - 3 : (u < 9) ? u - 5 : 3;
+      const int i = ti * 16 + (lane & 15), j = tj * 16 + (lane & 15);
+      d4_t acc = {0.0, 0.0, 0.0, 0.0};
+      for (int ks = 4 * ti; ks < 4 * tj + 4; ks++) {
+        const int k = 4 * ks + (lane >> 4);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(luu(M2, i, k), uval(M4, k, j), acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = ti * 16 + (lane >> 4) + 4 * v, c = tj * 16 + (lane & 15);
+        if (r <= c && c < nb) a.T[(size_t)r + (size_t)c * a.ldt] = -acc[v];
+      }
+    }
+  }
+  __syncthreads();
+  store_slab(X, RS, nb, nloc, a.A + r0, a.lda, nullptr);
+  exit_protocol64(a);
+}
+
+// ---------------------------------------------------------------------------
 // Cholesky-QR2 + Householder reconstruction for a panel taller than the
 // workgroups' LDS can hold: the same algorithm as panel_cqr_kernel, with each
 // workgroup's rows streamed through LDS in sub-slabs of a.sr rows once per
@@ -1218,7 +1791,7 @@ size_t panel_smem_bytes(int nb, int rs) {
 }  // namespace
 
 // persistent scratch size (doubles) the runtime allocates and zeroes once
-extern "C" size_t ld_panel_scratch_doubles(void) { return SCRATCH; }
+extern "C" size_t ld_panel_scratch_doubles(void) { return SCRATCH64; }
 
 // mode: 0 column-by-column Householder, 1 Cholesky-QR2 + reconstruction
 // (with the column-by-column fallback inside)
@@ -1353,6 +1926,87 @@ extern "C" void ld_panel_qr(int ml, int nb, double *A, int lda, double *T, int l
                             int ldq) {
   if (ld_failed()) return;  // an earlier allocation of this call failed
   panel_launch(ld_tune()->panel_mode, ml, nb, A, lda, T, ldt, Q, ldq, 0);
+}
+
+// 64-wide Cholesky-QR2 panel: workgroups, rows per workgroup (at least 64,
+// so that workgroup 0 holds the panel's top 64 rows) and the odd LDS column
+// stride of the slab plus the 64 top rows; false when the panel is too tall
+// for 256 such slabs
+static bool panel64_geom(int ml, int *G_, int *rp_, int *rs_) {
+  int rsmax = (LDS_DYN_BYTES / 8 - CQR64_TAIL - 2) / 64;
+  if (!(rsmax & 1)) rsmax--;
+  int rpmax = rsmax - 64;
+  if (rpmax > PNT) rpmax = PNT;  // the fallback moves one slab row per thread
+  if (rpmax < 64) return false;
+  int target = ld_tune()->panel_rows_cqr;
+  if (target < 64) target = 64;
+  if (target > rpmax) target = rpmax;
+  int G = (ml + target - 1) / target;
+  if (G > 256) G = 256;
+  int rp = (ml + G - 1) / G;
+  if (rp < 64) rp = 64;
+  if (rp > rpmax) return false;
+  G = (ml + rp - 1) / rp;
+  *G_ = G;
+  *rp_ = rp;
+  *rs_ = (rp + 64) | 1;
+  return true;
+}
+
+// the ml x nb panel (32 < nb <= 64, no explicit Q) in one Cholesky-QR2
+// launch; -1 (nothing launched) when it does not take the panel: panel mode
+// 0, ml < 64, or taller than its LDS slabs
+extern "C" int ld_panel_qr64(int ml, int nb, double *A, int lda, double *T, int ldt) {
+  if (ld_failed()) return 0;  // an earlier allocation of this call failed
+  if (nb <= PNB || nb > 64 || ml < 64 || ld_tune()->panel_mode != 1) return -1;
+  int G, rp, rs;
+  if (!panel64_geom(ml, &G, &rp, &rs)) return -1;
+  static int attr = 0, coop = 0, ncu = 0;
+  if (!attr) {
+    const hipError_t e = hipFuncSetAttribute((const void *)panel_cqr64_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_BYTES);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      attr = -1;
+    } else {
+      int dev = 0, per_cu = 0;
+      hipGetDevice(&dev);
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)panel_cqr64_kernel, PNT,
+                                                   LDS_DYN_BYTES);
+      const char *ev = getenv("LILA_PANEL_COOP");
+      coop = (ev && atoi(ev) == 1) || per_cu < 1 || ncu < 1;
+      attr = 1;
+    }
+  }
+  if (attr < 0 || (ncu > 0 && G > ncu)) return -1;
+  PanelArgs pa;
+  memset(&pa, 0, sizeof pa);
+  pa.ml = ml; pa.nb = nb; pa.A = A; pa.lda = lda; pa.T = T; pa.ldt = ldt;
+  pa.Q = nullptr; pa.ldq = 0; pa.G = G; pa.rp = rp; pa.rs = rs; pa.sr = rp;
+  pa.decline = 0; pa.batch = 0;
+  pa.scr = ld_panel_scratch();
+  int *flags = ld_device_flags();
+  pa.cnt = (unsigned *)(flags + 8);
+  pa.err = flags;
+  void *args[] = {&pa};
+  const size_t shb = sizeof(double) * (slab_doubles_h(64, rs) + CQR64_TAIL);
+  const int ph = ld_prof_begin("panel_cqr64_kernel");
+  hipError_t e;
+  if ((coop || ld_lanes() > 1) && G > 1)
+    e = hipLaunchCooperativeKernel((const void *)panel_cqr64_kernel, dim3(G), dim3(PNT), args, (unsigned)shb,
+                                   (hipStream_t)ld_stream());
+  else
+    e = hipLaunchKernel((const void *)panel_cqr64_kernel, dim3(G), dim3(PNT), args, shb,
+                        (hipStream_t)ld_stream());
+  ld_prof_end(ph, 2.0 * ml * nb * nb, 8.0 * (2.0 * ml * nb + (double)nb * nb));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fprintf(stderr, "lila_hip: 64-wide panel launch failed (G=%d rp=%d nb=%d): %s\n", G, rp, nb,
+            hipGetErrorString(e));
+    ld_set_info(-1);
+  }
+  return 0;
 }
 
 extern "C" void ld_panel_qr_exact(int ml, int nb, double *A, int lda, double *T, int ldt, double *Q,

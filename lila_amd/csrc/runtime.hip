@@ -51,7 +51,7 @@ struct State {
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
-  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64, 0, 0, 0, 0, 1, 0};
+  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64, 0, 64, 0, 0, 1, 0};
   int failed = 0;  // a workspace allocation of the current call failed
 };
 State g;
@@ -112,6 +112,8 @@ int ld_init(void) {
   if (e) g.tune.gemm_kmin = atoi(e);
   e = getenv("LILA_PANEL_MODE");
   if (e) g.tune.panel_mode = atoi(e) ? 1 : 0;
+  e = getenv("LILA_PANEL_WIDTH");  // as lila_set_panel_width
+  if (e && ((atoi(e) >= 8 && atoi(e) <= LD_PANEL_MAX) || atoi(e) == 64)) g.tune.panel_nb = atoi(e);
   // the static scratch of lila_dgels_batched_noalloc on the current device
   if (ld_batch_noalloc_setup() != 0) return -1;
   g.inited = 1;
@@ -465,9 +467,10 @@ int lila_set_panel_mode(int mode) {
 
 int lila_set_panel_width(int nb) {
   if (ld_init() != 0) return -1;
-  if (nb < 8 || nb > LD_PANEL_MAX) return -1;
-  const int old = (g.tune.panel_nb >= 8 && g.tune.panel_nb <= LD_PANEL_MAX) ? g.tune.panel_nb
-                                                                              : LD_PANEL_MAX;
+  if ((nb < 8 || nb > LD_PANEL_MAX) && nb != 64) return -1;
+  const int old = ((g.tune.panel_nb >= 8 && g.tune.panel_nb <= LD_PANEL_MAX) || g.tune.panel_nb == 64)
+                      ? g.tune.panel_nb
+                      : LD_PANEL_MAX;
   g.tune.panel_nb = nb;
   return old;
 }
