@@ -612,7 +612,7 @@ def main():
     # mt (T block) and nx (recursion cutoff) as lila_autotune picks them for
     # m = n = 16384 on MI355X (bench.py --autotune reproduces the choice)
     ap.add_argument("--mt", type=int, default=1024)
-    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-prof", action="store_true")

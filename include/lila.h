@@ -395,7 +395,7 @@ int lila_dgemm_emu(char transa, char transb, int m, int n, int k, double alpha, 
  * lila_dgeqrf_w03_recursive (VRTQ, FP64, panel dgeqr3, leaf hh) on an m x n
  * Uniform(-1,1) device matrix and, one setting at a time from the current
  * ones (best of `reps` runs each), picks the T block height mt from
- * {256, 512, 1024} and the recursion cutoff nx from {32, 64, 128, 256} (both
+ * {256, 512, 1024} and the recursion cutoff nx from {32, 64, ..., 2048} (both
  * in/out: read as the starting point, written with the choice, which the
  * caller then passes as mt and lila_param[3]), then the GEMM tile shapes,
  * split-K occupancy target, small-output tile thresholds, panel slab

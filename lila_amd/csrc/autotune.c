@@ -120,8 +120,8 @@ int lila_autotune(int m, int n, int reps, int *mt, int *nx, char *report, int le
         sweep(&c, "mt", &c.mt, v, 3);
     }
     {
-        static const int v[] = {32, 64, 128, 256};
-        sweep(&c, "nx", &c.nx, v, 4);
+        static const int v[] = {32, 64, 128, 256, 512, 1024, 2048};
+        sweep(&c, "nx", &c.nx, v, 7);
     }
     {
         /* the tile shape is a pair: sweep an index into the table */
