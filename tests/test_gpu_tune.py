@@ -17,7 +17,7 @@ def test_autotune_choice_and_parity(lila, ora):
     say("autotune 3000 x 768 from mt 256, nx 32")
     mt, nx, report = lila.autotune(3000, 768, reps=1, mt=256, nx=32)
     say(report)
-    assert mt in (256, 512, 1024) and nx in (32, 64, 128, 256)
+    assert mt in (256, 512, 1024) and nx in (32, 64, 128, 256, 512, 1024, 2048)
     assert "panel_nb" in report and "(plain form)" in report
     assert report.startswith("start mt 256 nx 32") and "chose mt %d nx %d" % (mt, nx) in report
     for (m, n, ii) in ((1500, 700, 0), (1213, 999, 13)):
