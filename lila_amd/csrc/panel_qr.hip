@@ -609,6 +609,45 @@ __device__ __forceinline__ void slab_mul_inv(double *X, int RS, int nloc, int nb
   slab_mul(X, RS, nloc, nb, s, MI);
 }
 
+// The Cholesky factor of a Gram G = I + E that lies within NEAR_ID of the
+// identity entrywise, without the column-by-column chain: with X the strict
+// upper part of E plus half its diagonal, R = I + X gives R^T R = G + X^T X,
+// and R^-1 = (I + X)^-1 = I - X + X^2 - ...  For n <= 64 columns ||X||_F <=
+// ||E||_F <= 64 NEAR_ID < 1e-9, so the residual X^T X and the dropped X^2
+// are below 1e-18, a hundredth of the rounding unit: R is the Cholesky
+// factor and I - X its inverse to working precision.  This is the second
+// Cholesky-QR pass of a panel whose first pass worked (G = Q1^T Q1 with
+// ||I - G|| ~ kappa^2 u); a Gram further from I takes the Cholesky routine.
+constexpr double NEAR_ID = 1e-11;
+__device__ __forceinline__ bool near_id(double g, bool diag) {
+  return fabs(diag ? g - 1.0 : g) <= NEAR_ID;  // false for NaN
+}
+
+// 32 x 32 form (row-major, ld LDM, upper part read, identity beyond the
+// panel width): R into M (zero below the diagonal), R^-1 into MI.  All
+// threads; returns false and writes nothing when G is not near I.
+__device__ bool near_identity32(double *M, double *MI, int *flagp) {
+  const int tid = threadIdx.x;
+  if (tid == 0) *flagp = 1;
+  __syncthreads();
+  for (int t = tid; t < PNB * PNB; t += PNT) {
+    const int r = t / PNB, c = t % PNB;
+    if (r <= c && !near_id(M[r * LDM + c], r == c)) *flagp = 0;
+  }
+  __syncthreads();
+  const bool ok = *flagp != 0;
+  if (ok) {
+    for (int t = tid; t < PNB * PNB; t += PNT) {
+      const int r = t / PNB, c = t % PNB;
+      const double g = M[r * LDM + c];
+      M[r * LDM + c] = (r < c) ? g : (r == c ? 0.5 * (1.0 + g) : 0.0);
+      MI[r * LDM + c] = (r < c) ? -g : (r == c ? 0.5 * (3.0 - g) : 0.0);
+    }
+  }
+  __syncthreads();
+  return ok;
+}
+
 // the fallback path out of line, so that its registers do not weigh on the
 // Cholesky-QR path's allocation
 __device__ __noinline__ void hh_fallback(const PanelArgs &a, double *slab, double *tail, int r0,
@@ -632,6 +671,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   int *flag = (int *)(D2 + PNB);  // [0] proceed, [1] the LU's verdict
   double *G1 = a.scr + OFF_G1, *G2 = a.scr + OFF_G2;
   unsigned nbar = 0;
+  bool near2 = false;  // pass 2's Gram within NEAR_ID of I: no Cholesky chain
 
   // the panel's top nb rows (X_top, column-major in M3 with stride LDM) go
   // through both passes beside the slab, with the same arithmetic as those
@@ -685,15 +725,19 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       M2[r * LDM + c] = v;
     }
     __syncthreads();
-    if (wave == 0) {
-      const bool ok = wave_chol32(M2, D2, nb);
-      if (tid == 0) flag[0] = ok ? 1 : 0;
+    near2 = near_identity32(M2, M4, &flag[3]);  // R2 and R2^-1 without the chain
+    if (!near2) {
+      if (wave == 0) {
+        const bool ok = wave_chol32(M2, D2, nb);
+        if (tid == 0) flag[0] = ok ? 1 : 0;
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (flag[0]) {
-    slab_mul_inv(X, RS, nloc, nb, M2, D2, nullptr, M4);  // X = Q
-    slab_mul(M3, LDM, nb, nb, nullptr, M4);              // X_top = Q_top
+    if (near2) slab_mul(X, RS, nloc, nb, nullptr, M4);      // X = Q
+    else slab_mul_inv(X, RS, nloc, nb, M2, D2, nullptr, M4);
+    slab_mul(M3, LDM, nb, nb, nullptr, M4);                 // X_top = Q_top
     if (g == 0) {
       // R = R2 R1 (both upper) on MFMA, one 16 x 16 tile per wave, into M0
       const int ti = wave & 1, tj = wave >> 1;
@@ -883,6 +927,33 @@ __device__ bool wg_chol64(double *M, double *D, double *Sc, int nb, int *flag) {
   }
   __syncthreads();
   return flag[0] != 0;
+}
+
+// near_identity32 for the 64 x 64 Gram in 3-block storage (upper part read,
+// identity beyond the panel width): R into M, R^-1 into MI (3 blocks each,
+// zero below the diagonal)
+__device__ bool near_identity64(double *M, double *MI, int *flagp) {
+  const int tid = threadIdx.x;
+  if (tid == 0) *flagp = 1;
+  __syncthreads();
+  for (int t = tid; t < 64 * 64; t += PNT) {
+    const int r = t >> 6, c = t & 63;
+    if (r <= c && !near_id(M[((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)], r == c)) *flagp = 0;
+  }
+  __syncthreads();
+  const bool ok = *flagp != 0;
+  if (ok) {
+    for (int t = tid; t < 64 * 64; t += PNT) {
+      const int r = t >> 6, c = t & 63;
+      if (r >= 32 && c < 32) continue;
+      const int o = ((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31);
+      const double g = M[o];
+      M[o] = (r < c) ? g : (r == c ? 0.5 * (1.0 + g) : 0.0);
+      MI[o] = (r < c) ? -g : (r == c ? 0.5 * (3.0 - g) : 0.0);
+    }
+  }
+  __syncthreads();
+  return ok;
 }
 
 // X = R^-1 for the upper 64 x 64 R given as its blocks R11, R12, R22 (lower
@@ -1227,6 +1298,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   int *flag = (int *)(one + 64);
   double *G1 = a.scr + OFF64_G1, *G2 = a.scr + OFF64_G2;
   unsigned nbar = 0;
+  bool near2 = false;  // pass 2's Gram within NEAR_ID of I: no Cholesky chain
 
   // slab and, at rows rp .., the panel's top 64 rows (see panel_cqr_kernel)
   for (int t = tid; t < 64 * 64; t += PNT) {
@@ -1250,10 +1322,11 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     nbar++;
     grid_sync(a.cnt, nbar * (unsigned)G, a.err);
     gram64_read(G2, nb, M2);
-    ok = wg_chol64(M2, D2, M4, nb, flag);
+    near2 = near_identity64(M2, M4, &flag[3]);
+    ok = near2 || wg_chol64(M2, D2, M4, nb, flag);
   }
   if (ok) {
-    wg_inv64(ublk(M2, 0, 0), ublk(M2, 0, 1), ublk(M2, 1, 1), D2, M4);
+    if (!near2) wg_inv64(ublk(M2, 0, 0), ublk(M2, 0, 1), ublk(M2, 1, 1), D2, M4);
     slab_mul64(X, RS, a.rp + 64, nb, nullptr, M4);  // Q, and Q_top at rows rp ..
     if (g == 0) {
       // R = R2 R1 into M0: the three upper 32 x 32 blocks, each from

@@ -152,6 +152,17 @@ __device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb) {
   return ok && dmin > 1e-5 * dmax;
 }
 
+// 1 / x for the pivots of the sign-chosen LU (|x| = |t| + 1 >= 1, finite):
+// the hardware estimate refined by two Newton steps, within an ulp of the
+// rounded quotient (the IEEE division sequence is twice as long a chain)
+__device__ __forceinline__ double recip(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
 // sign-chosen LU (see wave_lu_sign32) of the 16 x 16 block of M at (o, o)
 // with the signs left unapplied: U above the diagonal as eliminated, the
 // pivots on it, L below; S[o..], D[o..] = signs, 1 / pivots
@@ -169,14 +180,16 @@ __device__ __forceinline__ void lu16(double *M, double *S, double *D, int o, int
       if (fabs(t) > 1.0 - 1e-10) ok = false;
       const double sk = flip ? -1.0 : 1.0;
       const double piv = sk * t - 1.0;
-      const double pinv = 1.0 / piv;
+      const double pinv = recip(piv);
       const double f = sk * pinv;
       // U(k,c) for the trailing columns; finished columns c < k keep theirs
+      // (uk = 0), column k becomes L (scaled by f, the same product as l)
       const double uk = (c > k) ? a[k] : 0.0;
+      const double gk = (c == k) ? f : 1.0;
 #pragma unroll
       for (int r = k + 1; r < 16; r++) {
         const double l = rdlane(a[r], k) * f;  // L(r,k)
-        a[r] = (c == k) ? l : a[r] - l * uk;
+        a[r] = fma(-l, uk, a[r] * gk);
       }
       if (c == k) {
         a[k] = piv;

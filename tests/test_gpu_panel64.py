@@ -140,3 +140,30 @@ def test_recursive_with_panel64(lila, ora, m, n, ii, mt, nx):
     say("recursive with 64-wide panels", m, n, ii, mt, nx)
     A, o, g = run_driver(lila, ora, "recursive", m, n, ii, mt, [0, 1, 1, nx])
     check_vrtq(ora, A, o, g, m, n, ii, mt, m, m)
+
+
+@pytest.mark.parametrize("width", [64, 32])
+@pytest.mark.parametrize("kappa", [1e1, 1e3, 1e4])
+def test_second_pass_near_identity_or_cholesky(lila, ora, width, kappa):
+    """A = U diag(sigma) V^T with sigma graded from 1 to 1/kappa: after the
+    first Cholesky-QR pass the Gram of A R1^-1 is I + E with ||E|| ~ kappa^2 u,
+    within the near-identity bound (1e-11 entrywise) at kappa = 10, beyond it
+    at 1e3 and 1e4, where the second pass takes the Cholesky routine.  Both
+    sides give dgeqr3's V, R, T (differences within kappa n u) and an exact
+    factorization with orthonormal Q."""
+    lila.call("lila_set_panel_width", width)
+    m, n = 6000, width
+    U, _ = np.linalg.qr(ora.fill_normal(31, m, n))
+    W, _ = np.linalg.qr(ora.fill_normal(32, n, n))
+    sig = kappa ** (-np.arange(n) / (n - 1))
+    A = np.asfortranarray((U * sig) @ W.T)
+    say("second pass, width", width, "kappa", kappa)
+    (Ao, To), (Ag, Tg), n64 = run_dgeqr3(lila, ora, A)
+    assert n64 == (1 if width == 64 else 0)
+    tol = 50 * n * U64 * kappa
+    assert rel(Ag, Ao) < tol
+    assert rel(np.triu(Tg), np.triu(To)) < tol
+    V = np.tril(Ag, -1) + np.eye(m, n)
+    Qx = np.eye(m, n) - V @ (np.triu(Tg) @ V[:n].T)
+    assert np.linalg.norm(Qx @ np.triu(Ag[:n]) - A) / np.linalg.norm(A) < 100 * n * U64
+    assert np.linalg.norm(Qx.T @ Qx - np.eye(n)) < 100 * n * U64
