@@ -63,6 +63,7 @@ __device__ __forceinline__ d4s_t mm16(const double *A, int ars, int acs, const d
 __device__ __forceinline__ void chol16(double *M, double *D, int o, int nb, bool &ok, double &dmin,
                                        double &dmax) {
   const int lane = opaque_lane(), c = lane & 15;
+  nb = __builtin_amdgcn_readfirstlane(nb);  // scalar branches (see lu16)
   double a[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) a[r] = M[(o + r) * LDM + o + c];
@@ -85,7 +86,7 @@ __device__ __forceinline__ void chol16(double *M, double *D, int o, int nb, bool
       dmax = fmax(dmax, p);
       dmin = fmin(dmin, p);
     }
-    if (c == j) myinv = pinv;
+    myinv = (c == j) ? pinv : myinv;
     const double rj = (c == j) ? p : a[j] * pinv;
     a[j] = rj;
 #pragma unroll
@@ -174,29 +175,30 @@ __device__ __forceinline__ void lu16(double *M, double *S, double *D, int o, int
   double mys = 1.0, myinv = 1.0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    if (o + k < nb) {
-      const double t = rdlane(a[k], k);
-      const bool flip = fabs(1.0 - t) < fabs(1.0 + t);
-      if (fabs(t) > 1.0 - 1e-10) ok = false;
-      const double sk = flip ? -1.0 : 1.0;
-      const double piv = sk * t - 1.0;
-      const double pinv = recip(piv);
-      const double f = sk * pinv;
-      // U(k,c) for the trailing columns; finished columns c < k keep theirs
-      // (uk = 0), column k becomes L (scaled by f, the same product as l)
-      const double uk = (c > k) ? a[k] : 0.0;
-      const double gk = (c == k) ? f : 1.0;
+    // Steps k >= nb (identity columns: t = 1, zeros below) run too, with
+    // their effect masked by act, so that no step is a branch: a branch
+    // makes the compiler move a[] between register sets at every step.
+    const bool act = o + k < nb;
+    const double t = rdlane(a[k], k);
+    const bool flip = fabs(1.0 - t) < fabs(1.0 + t);
+    if (act && fabs(t) > 1.0 - 1e-10) ok = false;
+    const double sk = flip ? -1.0 : 1.0;
+    const double piv = sk * t - 1.0;  // |piv| = |t| + 1 >= 1
+    const double pinv = recip(piv);
+    const double f = act ? sk * pinv : 0.0;
+    // U(k,c) for the trailing columns; finished columns c < k keep theirs
+    // (uk = 0), column k becomes L (scaled by f, the same product as l)
+    const bool mine = act && c == k;
+    const double uk = (c > k) ? a[k] : 0.0;
+    const double gk = mine ? f : 1.0;
 #pragma unroll
-      for (int r = k + 1; r < 16; r++) {
-        const double l = rdlane(a[r], k) * f;  // L(r,k)
-        a[r] = fma(-l, uk, a[r] * gk);
-      }
-      if (c == k) {
-        a[k] = piv;
-        mys = sk;
-        myinv = pinv;
-      }
+    for (int r = k + 1; r < 16; r++) {
+      const double l = rdlane(a[r], k) * f;  // L(r,k)
+      a[r] = fma(-l, uk, a[r] * gk);
     }
+    a[k] = mine ? piv : a[k];
+    mys = mine ? sk : mys;
+    myinv = mine ? pinv : myinv;
   }
   if (lane < 16) {
 #pragma unroll
