@@ -609,6 +609,23 @@ __device__ __forceinline__ void slab_mul_inv(double *X, int RS, int nloc, int nb
   slab_mul(X, RS, nloc, nb, s, MI);
 }
 
+// the reduced 32 x 32 Gram (upper part, first nb) into M (row-major, ld
+// LDM), identity beyond nb, zero below the diagonal; all of a thread's
+// loads are issued before the first is used (one round trip to the
+// coherence point, not one per entry)
+__device__ void gram32_read(const double *G, int nb, double *M) {
+  constexpr int PER = PNB * PNB / PNT;
+  double g[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) g[u] = ld_coh(&G[threadIdx.x + u * PNT]);
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int t = threadIdx.x + u * PNT, r = t / PNB, c = t % PNB;
+    M[r * LDM + c] = (r < nb && c < nb) ? (r <= c ? g[u] : 0.0) : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+}
+
 // The Cholesky factor of a Gram G = I + E that lies within NEAR_ID of the
 // identity entrywise, without the column-by-column chain: with X the strict
 // upper part of E plus half its diagonal, R = I + X gives R^T R = G + X^T X,
@@ -692,14 +709,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   nbar++;
   grid_sync(a.cnt, nbar * (unsigned)G, a.err);
-  for (int t = tid; t < PNB * PNB; t += PNT) {
-    const int r = t / PNB, c = t % PNB;
-    double v = 0.0;
-    if (r < nb && c < nb) { if (r <= c) v = ld_coh(&G1[r * PNB + c]); }
-    else if (r == c) v = 1.0;
-    M0[r * LDM + c] = v;
-  }
-  __syncthreads();
+  gram32_read(G1, nb, M0);
   if (wave == 0) {
     const bool ok = wave_chol32(M0, D0, nb);
     if (tid == 0) flag[0] = ok ? 1 : 0;
@@ -717,14 +727,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     }
     nbar++;
     grid_sync(a.cnt, nbar * (unsigned)G, a.err);
-    for (int t = tid; t < PNB * PNB; t += PNT) {
-      const int r = t / PNB, c = t % PNB;
-      double v = 0.0;
-      if (r < nb && c < nb) { if (r <= c) v = ld_coh(&G2[r * PNB + c]); }
-      else if (r == c) v = 1.0;
-      M2[r * LDM + c] = v;
-    }
-    __syncthreads();
+    gram32_read(G2, nb, M2);
     near2 = near_identity32(M2, M4, &flag[3]);  // R2 and R2^-1 without the chain
     if (!near2) {
       if (wave == 0) {
@@ -1051,12 +1054,18 @@ __device__ void slab_gram64_add(const double *X, int RS, int nloc, int nb, doubl
 // the reduced Gram (upper, first nb) into M (3-block storage), identity
 // beyond nb, zero below the diagonal
 __device__ void gram64_read(const double *G, int nb, double *M) {
-  for (int t = threadIdx.x; t < 64 * 64; t += PNT) {
-    const int r = t >> 6, c = t & 63;
+  // all of a thread's loads are issued before the first is used (one
+  // round trip to the coherence point, not one per entry): every entry of
+  // G is loaded, the ones not needed are then dropped
+  constexpr int PER = 64 * 64 / PNT;
+  double g[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) g[u] = ld_coh(&G[threadIdx.x + u * PNT]);
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int t = threadIdx.x + u * PNT, r = t >> 6, c = t & 63;
     if (r >= 32 && c < 32) continue;
-    double v = 0.0;
-    if (r < nb && c < nb) { if (r <= c) v = ld_coh(&G[r * 64 + c]); }
-    else if (r == c) v = 1.0;
+    const double v = (r < nb && c < nb) ? (r <= c ? g[u] : 0.0) : (r == c ? 1.0 : 0.0);
     M[((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)] = v;
   }
   __syncthreads();
@@ -1327,41 +1336,46 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   if (ok) {
     if (!near2) wg_inv64(ublk(M2, 0, 0), ublk(M2, 0, 1), ublk(M2, 1, 1), D2, M4);
-    slab_mul64(X, RS, a.rp + 64, nb, nullptr, M4);  // Q, and Q_top at rows rp ..
+    // only the top rows become Q (Q_top = X_top R2^-1, for the LU): the
+    // slab keeps X = A R1^-1, and V2 is formed from it below with the
+    // combined transform R2^-1 S U^-1 (one pass over the slab, not two)
+    slab_mul64(X + a.rp, RS, 64, nb, nullptr, M4);
+    // R = R2 R1 (workgroup 0 only), from registers after all reads
+    const int lane = tid & 63, wave = tid >> 6;
+    const int ti = wave & 1, tj = wave >> 1;
+    d4_t racc[3];
     if (g == 0) {
-      // R = R2 R1 into M0: the three upper 32 x 32 blocks, each from
-      // registers after all reads
-      const int lane = tid & 63, wave = tid >> 6;
-      const int ti = wave & 1, tj = wave >> 1;
-      d4_t acc[3];
 #pragma unroll
       for (int bq = 0; bq < 3; bq++) {
         const int bi = (bq == 2) ? 1 : 0, bj = (bq == 0) ? 0 : 1;
-        acc[bq] = d4_t{0.0, 0.0, 0.0, 0.0};
+        racc[bq] = d4_t{0.0, 0.0, 0.0, 0.0};
         const int i = 32 * bi + ti * 16 + (lane & 15), j = 32 * bj + tj * 16 + (lane & 15);
         for (int ks = 8 * bi; ks < 8 * bj + 8; ks++) {
           const int k = 4 * ks + (lane >> 4);
-          acc[bq] = __builtin_amdgcn_mfma_f64_16x16x4f64(uval(M2, i, k), uval(M0, k, j), acc[bq], 0, 0, 0);
+          racc[bq] = __builtin_amdgcn_mfma_f64_16x16x4f64(uval(M2, i, k), uval(M0, k, j), racc[bq], 0, 0, 0);
         }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int bq = 0; bq < 3; bq++) {
-        const int bi = (bq == 2) ? 1 : 0, bj = (bq == 0) ? 0 : 1;
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          ublk(M0, bi, bj)[(ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15)] = acc[bq][v];
       }
     }
     __syncthreads();
-    // Q_top into M2 (4 blocks, identity beyond nb), its sign-chosen LU
+    // Q_top into M2 (4 blocks, identity beyond nb)
     for (int t = tid; t < 64 * 64; t += PNT) {
       const int r = t >> 6, c = t & 63;
       fblk(M2, r >> 5, c >> 5)[(r & 31) * LDM + (c & 31)] =
           (r < nb && c < nb) ? X[a.rp + r + c * RS] : (r == c ? 1.0 : 0.0);
     }
     __syncthreads();
-    ok = wg_lu64(M2, Sv, D2, M4, one, nb, flag);
+    if (g == 0) {
+      // R into the X_top rows (free now), column-major like the slab
+#pragma unroll
+      for (int bq = 0; bq < 3; bq++) {
+        const int bi = (bq == 2) ? 1 : 0, bj = (bq == 0) ? 0 : 1;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          X[a.rp + 32 * bi + ti * 16 + (lane >> 4) + 4 * v + (32 * bj + tj * 16 + (lane & 15)) * RS] = racc[bq][v];
+      }
+    }
+    // the sign-chosen LU of Q_top (M0 free in every workgroup: scratch)
+    ok = wg_lu64(M2, Sv, D2, M0, one, nb, flag);
   }
 
   if (!ok) {
@@ -1371,14 +1385,41 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     return;
   }
 
-  // ---- outputs: V2 = (Q2 S) U^-1 ----
-  wg_inv64(fblk(M2, 0, 0), fblk(M2, 0, 1), fblk(M2, 1, 1), D2, M4);
-  slab_mul64(X, RS, nloc, nb, Sv, M4);
+  // ---- outputs: V2 = X (R2^-1 S U^-1) = (Q2 S) U^-1 ----
+  wg_inv64(fblk(M2, 0, 0), fblk(M2, 0, 1), fblk(M2, 1, 1), D2, M0);  // U^-1
+  {
+    // M4 <- M4 diag(S) M0 (upper x diag x upper), from registers after all reads
+    const int lane = tid & 63, wave = tid >> 6;
+    const int ti = wave & 1, tj = wave >> 1;
+    d4_t macc[3];
+#pragma unroll
+    for (int bq = 0; bq < 3; bq++) {
+      const int bi = (bq == 2) ? 1 : 0, bj = (bq == 0) ? 0 : 1;
+      macc[bq] = d4_t{0.0, 0.0, 0.0, 0.0};
+      const int i = 32 * bi + ti * 16 + (lane & 15), j = 32 * bj + tj * 16 + (lane & 15);
+      for (int ks = 8 * bi; ks < 8 * bj + 8; ks++) {
+        const int k = 4 * ks + (lane >> 4);
+        const double sk = (k < nb) ? Sv[k] : 1.0;
+        macc[bq] = __builtin_amdgcn_mfma_f64_16x16x4f64(uval(M4, i, k) * sk, uval(M0, k, j), macc[bq], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bq = 0; bq < 3; bq++) {
+      const int bi = (bq == 2) ? 1 : 0, bj = (bq == 0) ? 0 : 1;
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        ublk(M4, bi, bj)[(ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15)] = macc[bq][v];
+    }
+    __syncthreads();
+  }
+  slab_mul64(X, RS, nloc, nb, nullptr, M4);
   if (g == 0) {
-    // top nb rows: R' = S R2 R1 on and above the diagonal, V1 = L below
+    // top nb rows: R' = S R2 R1 on and above the diagonal (R from the
+    // X_top rows), V1 = L below
     for (int t = tid; t < nb * nb; t += PNT) {
       const int r = t % nb, c = t / nb;
-      X[r + c * RS] = (r <= c) ? Sv[r] * uval(M0, r, c) : lul(M2, r, c);
+      X[r + c * RS] = (r <= c) ? Sv[r] * X[a.rp + r + c * RS] : lul(M2, r, c);
     }
   }
   if (g == G - 1) {
@@ -1459,14 +1500,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   };
   // Gram from global into M (identity beyond nb) and its Cholesky
   auto chol_of = [&](const double *Gd, double *M, double *D) {
-    for (int t = tid; t < PNB * PNB; t += PNT) {
-      const int r = t / PNB, c = t % PNB;
-      double v = 0.0;
-      if (r < nb && c < nb) { if (r <= c) v = ld_coh(&Gd[r * PNB + c]); }
-      else if (r == c) v = 1.0;
-      M[r * LDM + c] = v;
-    }
-    __syncthreads();
+    gram32_read(Gd, nb, M);
     if (wave == 0) {
       const bool ok = wave_chol32(M, D, nb);
       if (tid == 0) flag[0] = ok ? 1 : 0;
