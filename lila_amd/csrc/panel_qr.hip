@@ -178,11 +178,28 @@ __device__ __forceinline__ size_t slab_doubles(int nb, int rs) { return ((size_t
 // the slab's nloc x nb entries, every thread taking entries nloc apart
 // (loads coalesced over the rows, all of a thread's loads independent)
 __device__ void load_slab(const PanelArgs &a, double *slab, int r0, int nloc) {
-  const int tot = nloc * a.nb;
-#pragma unroll 4
-  for (int e = threadIdx.x; e < tot; e += PNT) {
-    const int lr = e % nloc, c = e / nloc;
-    slab[lr + c * a.rs] = a.A[(size_t)(r0 + lr) + (size_t)c * a.lda];
+  // a wave's lanes run down the rows of a column (coalesced); each round
+  // issues the loads of 8 columns x 2 blocks of 64 rows before its first
+  // LDS write, so the round trips to HBM overlap
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nb = a.nb;
+  for (int rb0 = 0; rb0 < nloc; rb0 += 128) {
+    for (int c0 = wave; c0 < nb; c0 += 32) {
+      double v[8][2];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int c = c0 + 4 * q, lr = rb0 + 64 * h + lane;
+          v[q][h] = (c < nb && lr < nloc) ? a.A[(size_t)(r0 + lr) + (size_t)c * a.lda] : 0.0;
+        }
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int c = c0 + 4 * q, lr = rb0 + 64 * h + lane;
+          if (c < nb && lr < nloc) slab[lr + c * a.rs] = v[q][h];
+        }
+    }
   }
   __syncthreads();
 }
@@ -932,31 +949,41 @@ __device__ bool wg_chol64(double *M, double *D, double *Sc, int nb, int *flag) {
   return flag[0] != 0;
 }
 
-// near_identity32 for the 64 x 64 Gram in 3-block storage (upper part read,
-// identity beyond the panel width): R into M, R^-1 into MI (3 blocks each,
-// zero below the diagonal)
-__device__ bool near_identity64(double *M, double *MI, int *flagp) {
-  const int tid = threadIdx.x;
-  if (tid == 0) *flagp = 1;
-  __syncthreads();
-  for (int t = tid; t < 64 * 64; t += PNT) {
-    const int r = t >> 6, c = t & 63;
-    if (r <= c && !near_id(M[((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)], r == c)) *flagp = 0;
+// gram64_read and near_identity32's test and factor in one pass for the
+// 64 x 64 Gram: from the reduced G (upper, first nb; identity beyond), R =
+// I + X into M and R^-1 = I - X into MI (3-block storage, zero below the
+// diagonal) when every entry lies within NEAR_ID of the identity's;
+// otherwise G into M as gram64_read does.  Returns whether G was near I.
+// *flagp (LDS) must be 0 on entry, set before a barrier every thread passed.
+__device__ bool gram64_read_near(const double *G, int nb, double *M, double *MI, int *flagp) {
+  constexpr int PER = 64 * 64 / PNT;
+  double g[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) g[u] = ld_coh(&G[threadIdx.x + u * PNT]);
+  int bad = 0;
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int t = threadIdx.x + u * PNT, r = t >> 6, c = t & 63;
+    g[u] = (r < nb && c < nb) ? (r <= c ? g[u] : 0.0) : (r == c ? 1.0 : 0.0);
+    if (r <= c && !near_id(g[u], r == c)) bad = 1;
   }
+  if (bad) *flagp = 1;
   __syncthreads();
-  const bool ok = *flagp != 0;
-  if (ok) {
-    for (int t = tid; t < 64 * 64; t += PNT) {
-      const int r = t >> 6, c = t & 63;
-      if (r >= 32 && c < 32) continue;
-      const int o = ((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31);
-      const double g = M[o];
-      M[o] = (r < c) ? g : (r == c ? 0.5 * (1.0 + g) : 0.0);
-      MI[o] = (r < c) ? -g : (r == c ? 0.5 * (3.0 - g) : 0.0);
+  const bool near = *flagp == 0;
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int t = threadIdx.x + u * PNT, r = t >> 6, c = t & 63;
+    if (r >= 32 && c < 32) continue;
+    const int o = ((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31);
+    if (near) {
+      M[o] = (r < c) ? g[u] : (r == c ? 0.5 * (1.0 + g[u]) : 0.0);
+      MI[o] = (r < c) ? -g[u] : (r == c ? 0.5 * (3.0 - g[u]) : 0.0);
+    } else {
+      M[o] = g[u];
     }
   }
   __syncthreads();
-  return ok;
+  return near;
 }
 
 // X = R^-1 for the upper 64 x 64 R given as its blocks R11, R12, R22 (lower
@@ -1310,9 +1337,21 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   bool near2 = false;  // pass 2's Gram within NEAR_ID of I: no Cholesky chain
 
   // slab and, at rows rp .., the panel's top 64 rows (see panel_cqr_kernel)
-  for (int t = tid; t < 64 * 64; t += PNT) {
-    const int r = t & 63, c = t >> 6;
-    if (c < nb) X[a.rp + r + c * RS] = (r < ntop) ? a.A[(size_t)r + (size_t)c * a.lda] : 0.0;
+  {
+    // wave w loads columns w, w + 4, ..., lanes down the rows; all sixteen
+    // loads in flight before the first LDS write
+    const int lane = tid & 63, wave = tid >> 6;
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int c = wave + 4 * u;
+      v[u] = (c < nb && lane < ntop) ? a.A[(size_t)lane + (size_t)c * a.lda] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int c = wave + 4 * u;
+      if (c < nb) X[a.rp + lane + c * RS] = v[u];
+    }
   }
   if (tid < 64) one[tid] = 1.0;
   load_slab(a, X, r0, nloc);
@@ -1328,10 +1367,10 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     slab_mul64(X, RS, a.rp + 64, nb, nullptr, M4);
     // ---- pass 2 ----
     slab_gram64_add(X, RS, nloc, nb, G2);
+    if (tid == 0) flag[3] = 0;  // gram64_read_near's verdict (the barrier orders it)
     nbar++;
     grid_sync(a.cnt, nbar * (unsigned)G, a.err);
-    gram64_read(G2, nb, M2);
-    near2 = near_identity64(M2, M4, &flag[3]);
+    near2 = gram64_read_near(G2, nb, M2, M4, &flag[3]);
     ok = near2 || wg_chol64(M2, D2, M4, nb, flag);
   }
   if (ok) {
@@ -2074,6 +2113,8 @@ extern "C" int ld_panel_qr64(int ml, int nb, double *A, int lda, double *T, int 
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_BYTES);
     if (e != hipSuccess) {
       (void)hipGetLastError();
+      fprintf(stderr, "lila_hip: the 64-wide panel kernel cannot take %d bytes of LDS (%s); "
+              "64-wide panels take the 32-wide recursion\n", (int)LDS_DYN_BYTES, hipGetErrorString(e));
       attr = -1;
     } else {
       int dev = 0, per_cu = 0;
