@@ -88,7 +88,8 @@ struct PanelArgs {
   int G, rp, rs;      // workgroups, rows per slab, LDS column stride
   int sr;             // streamed panel: rows per LDS sub-slab
   double *scr;        // persistent global scratch (SCRATCH doubles)
-  unsigned *cnt;      // [0] arrivals, [1] exits
+  unsigned *cnt;      // [1] exits
+  unsigned *bar;      // grid barrier counters (in the persistent scratch)
   int *err;           // [0] barrier time-out, [2] a streamed panel declined
   int decline;        // streamed panel: leave a panel that fails the acceptance
                       // tests untouched and raise err[2] (no column loop)
@@ -120,18 +121,37 @@ __device__ __forceinline__ void max_coh(double *p, double v) {  // v >= 0
 // fences are needed: the arrival is issued only after this workgroup's
 // atomics completed, and the reads of the reduced values are issued only
 // after the poll saw every arrival.
+// The arrivals go through a two-level tree: atomics on one counter are
+// performed one after another, a few hundred workgroups on one counter cost
+// microseconds.  Workgroup g adds to the counter of its group g % 8, each
+// counter 4 KiB from the others (different memory channels); the group's
+// last arriver of the round (each round adds exactly the group's size, and
+// a round's arrivals all precede the next round's) adds one to bar[0],
+// which the waiters poll for one arrival per nonempty group and round.
+// The last workgroup out of a launch resets them all (exit protocols).
 // grid_sync in two halves, so a workgroup can work between its arrival and
 // the wait (LDS-only work: its global writes before arrive are the ones
 // the others see)
-__device__ void grid_arrive(unsigned *cnt) {
+constexpr int BAR_GROUPS = 8;
+constexpr int BAR_STRIDE = 1024;  // unsigned ints between two counters (4 KiB)
+__device__ void grid_arrive(unsigned *bar) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(&cnt[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ void grid_wait(unsigned *cnt, unsigned target, int *err) {
   if (threadIdx.x == 0) {
+    const unsigned G = gridDim.x, k = blockIdx.x % BAR_GROUPS;
+    const unsigned gs = (G - k + BAR_GROUPS - 1) / BAR_GROUPS;  // workgroups of group k
+    const unsigned old = __hip_atomic_fetch_add(&bar[BAR_STRIDE * (k + 1)], 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    if ((old + 1) % gs == 0) __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+// target: the barrier's round times the grid size (rounds count from 1)
+__device__ void grid_wait(unsigned *bar, unsigned target, int *err) {
+  if (threadIdx.x == 0) {
+    const unsigned G = gridDim.x, groups = G < BAR_GROUPS ? G : BAR_GROUPS;
+    const unsigned want = (target / G) * groups;
     unsigned spins = 0;
-    while (__hip_atomic_load(&cnt[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > (1u << 24)) {
         atomicOr(err, 1);
@@ -142,21 +162,15 @@ __device__ void grid_wait(unsigned *cnt, unsigned target, int *err) {
   __syncthreads();
 }
 
-__device__ void grid_sync(unsigned *cnt, unsigned target, int *err) {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(&cnt[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned spins = 0;
-    while (__hip_atomic_load(&cnt[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > (1u << 24)) {
-        atomicOr(err, 1);
-        break;
-      }
-    }
-  }
-  __syncthreads();
+__device__ void grid_sync(unsigned *bar, unsigned target, int *err) {
+  grid_arrive(bar);
+  grid_wait(bar, target, err);
+}
+
+// the exit protocols' reset of the barrier counters (last workgroup out)
+__device__ void reset_barrier(unsigned *bar) {
+  for (int k = 0; k <= BAR_GROUPS; k++)
+    __hip_atomic_store(&bar[BAR_STRIDE * k], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -312,7 +326,7 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
       __syncthreads();
     } else {
       nbar++;
-      grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+      grid_sync(a.bar, nbar * (unsigned)G, a.err);
     }
 
     // ---- read the reduced vector ----
@@ -371,7 +385,7 @@ __device__ void hh_factor(const PanelArgs &a, double *slab, double *tail, int r0
           __syncthreads();
         } else {
           nbar++;
-          grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+          grid_sync(a.bar, nbar * (unsigned)G, a.err);
         }
         if (tid == 0) {
           scal[3] = sc * sqrt(SOLO ? rv[SLOT_SCALED] : ld_coh(&rv[SLOT_SCALED]));
@@ -513,7 +527,7 @@ __device__ void exit_protocol(const PanelArgs &a) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
-      __hip_atomic_store(&a.cnt[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      reset_barrier(a.bar);
       __hip_atomic_store(&a.cnt[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
@@ -725,7 +739,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     if (r <= c) add_coh(&G1[r * PNB + c], M0[r * LDM + c]);
   }
   nbar++;
-  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  grid_sync(a.bar, nbar * (unsigned)G, a.err);
   gram32_read(G1, nb, M0);
   if (wave == 0) {
     const bool ok = wave_chol32(M0, D0, nb);
@@ -743,7 +757,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       if (r <= c) add_coh(&G2[r * PNB + c], M2[r * LDM + c]);
     }
     nbar++;
-    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    grid_sync(a.bar, nbar * (unsigned)G, a.err);
     gram32_read(G2, nb, M2);
     near2 = near_identity32(M2, M4, &flag[3]);  // R2 and R2^-1 without the chain
     if (!near2) {
@@ -862,6 +876,9 @@ constexpr int OFF64_G2 = OFF64_G1 + 4096;       // pass 2
 constexpr int OFF64_W = OFF64_G2 + 4096;        // fallback: V1^T A2 (32 x 32)
 constexpr int OFF64_H = OFF64_W + 1024;         // fallback: V1^T V2 (32 x 32)
 constexpr int SCRATCH64 = OFF64_H + 1024;
+// the grid barrier's counters: 1 + BAR_GROUPS unsigned ints, 4 KiB apart
+constexpr int OFF_BAR = ((SCRATCH64 + 511) / 512) * 512;
+constexpr int SCRATCH_ALL = OFF_BAR + 512 * (BAR_GROUPS + 1);
 // LDS past the slab: M0 (3 blocks), M2 (4), M4 (3), S, D0, D2, ones (64
 // each), flags
 constexpr int CQR64_TAIL = 10 * B32 + 4 * 64 + 4;
@@ -1154,7 +1171,7 @@ __device__ void exit_protocol64(const PanelArgs &a) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
-      __hip_atomic_store(&a.cnt[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      reset_barrier(a.bar);
       __hip_atomic_store(&a.cnt[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
@@ -1208,7 +1225,7 @@ __device__ __noinline__ void hh_fallback64(const PanelArgs &a, int tail_off, int
     }
   }
   nbar++;
-  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  grid_sync(a.bar, nbar * (unsigned)G, a.err);
   for (int t = tid; t < PNB * PNB; t += PNT) Wl[(t / PNB) * LDM + t % PNB] = ld_coh(&W[t]);
   __syncthreads();
   // W' = T1^T W (T1 upper), into Wl
@@ -1296,7 +1313,7 @@ __device__ __noinline__ void hh_fallback64(const PanelArgs &a, int tail_off, int
     }
   }
   nbar++;
-  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  grid_sync(a.bar, nbar * (unsigned)G, a.err);
   if (g == 0) {
     // T2 (the column loop's T of the right columns, in LDS after it) and H
     const double *T2 = tail + PNB * PNB;  // the column loop's T, T2[r + PNB c]
@@ -1359,7 +1376,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   // ---- pass 1 ----
   slab_gram64_add(X, RS, nloc, nb, G1);
   nbar++;
-  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  grid_sync(a.bar, nbar * (unsigned)G, a.err);
   gram64_read(G1, nb, M0);
   bool ok = wg_chol64(M0, D0, M4, nb, flag);
   if (ok) {
@@ -1369,7 +1386,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     slab_gram64_add(X, RS, nloc, nb, G2);
     if (tid == 0) flag[3] = 0;  // gram64_read_near's verdict (the barrier orders it)
     nbar++;
-    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    grid_sync(a.bar, nbar * (unsigned)G, a.err);
     near2 = gram64_read_near(G2, nb, M2, M4, &flag[3]);
     ok = near2 || wg_chol64(M2, D2, M4, nb, flag);
   }
@@ -1565,7 +1582,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   // ---- pass 1: G1 = A^T A, R1 ----
   gram_pass(G1, nullptr, M0);
   nbar++;
-  grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+  grid_sync(a.bar, nbar * (unsigned)G, a.err);
   chol_of(G1, M0, D0);
   if (flag[0]) {
     // ---- pass 2: G2 = (A R1^-1)^T (A R1^-1), R2 ----
@@ -1573,7 +1590,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     __syncthreads();
     gram_pass(G2, M4, M2);
     nbar++;
-    grid_sync(a.cnt, nbar * (unsigned)G, a.err);
+    grid_sync(a.bar, nbar * (unsigned)G, a.err);
     chol_of(G2, M2, D2);
   }
   if (flag[0]) {
@@ -1603,9 +1620,9 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       if (tid == 0) st_coh(&pub[PNB * PNB + PNB], flag[1] ? 1.0 : 0.0);
     }
     nbar++;
-    grid_arrive(a.cnt);
+    grid_arrive(a.bar);
     if (g == 0) upper_mul(M2, M0, M0);  // R = R2 R1 while the others wait
-    grid_wait(a.cnt, nbar * (unsigned)G, a.err);
+    grid_wait(a.bar, nbar * (unsigned)G, a.err);
     if (tid == 0) flag[0] = ld_coh(&pub[PNB * PNB + PNB]) != 0.0 ? 1 : 0;
     __syncthreads();
   }
@@ -1937,7 +1954,7 @@ size_t panel_smem_bytes(int nb, int rs) {
 }  // namespace
 
 // persistent scratch size (doubles) the runtime allocates and zeroes once
-extern "C" size_t ld_panel_scratch_doubles(void) { return SCRATCH64; }
+extern "C" size_t ld_panel_scratch_doubles(void) { return SCRATCH_ALL; }
 
 // mode: 0 column-by-column Householder, 1 Cholesky-QR2 + reconstruction
 // (with the column-by-column fallback inside)
@@ -2026,6 +2043,7 @@ static void panel_launch(int mode, int ml, int nb, double *A, int lda, double *T
   pa.scr = ld_panel_scratch();
   int *flags = ld_device_flags();
   pa.cnt = (unsigned *)(flags + 8);
+  pa.bar = (unsigned *)(pa.scr + OFF_BAR);
   pa.err = flags;
   const void *fn;
   const char *cls;
@@ -2136,6 +2154,7 @@ extern "C" int ld_panel_qr64(int ml, int nb, double *A, int lda, double *T, int 
   pa.scr = ld_panel_scratch();
   int *flags = ld_device_flags();
   pa.cnt = (unsigned *)(flags + 8);
+  pa.bar = (unsigned *)(pa.scr + OFF_BAR);
   pa.err = flags;
   void *args[] = {&pa};
   const size_t shb = sizeof(double) * (slab_doubles_h(64, rs) + CQR64_TAIL);
