@@ -172,7 +172,6 @@ __device__ __forceinline__ void lu16(double *M, double *S, double *D, int o, int
   double a[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) a[r] = M[(o + r) * LDM + o + c];
-  double mys = 1.0, myinv = 1.0;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     // Steps k >= nb (identity columns: t = 1, zeros below) run too, with
@@ -180,31 +179,47 @@ __device__ __forceinline__ void lu16(double *M, double *S, double *D, int o, int
     // makes the compiler move a[] between register sets at every step.
     const bool act = o + k < nb;
     const double t = rdlane(a[k], k);
-    const bool flip = fabs(1.0 - t) < fabs(1.0 + t);
+    // the rule |1 - t| < |1 + t| (S = -1) holds exactly when t > 0; the
+    // pivot S t - 1 is then -(1 + |t|) either way, so the chain from t to
+    // the multiplier is one add, the reciprocal and a sign
+    const bool flip = t > 0.0;
     if (act && fabs(t) > 1.0 - 1e-10) ok = false;
     const double sk = flip ? -1.0 : 1.0;
-    const double piv = sk * t - 1.0;  // |piv| = |t| + 1 >= 1
-    const double pinv = recip(piv);
-    const double f = act ? sk * pinv : 0.0;
-    // U(k,c) for the trailing columns; finished columns c < k keep theirs
-    // (uk = 0), column k becomes L (scaled by f, the same product as l)
-    const bool mine = act && c == k;
-    const double uk = (c > k) ? a[k] : 0.0;
-    const double gk = mine ? f : 1.0;
+    const double ap = 1.0 + fabs(t);
+    const double piv = -ap;
+    const double rp = recip(ap);
+    const double pinv = -rp;
+    const double f = act ? (flip ? rp : -rp) : 0.0;  // S / pivot
+    // the step's sign and reciprocal pivot are the same on every lane:
+    // stored by all (column k reads them back at the end); past nb the
+    // identity's 1, 1
+    S[o + k] = act ? sk : 1.0;
+    D[o + k] = act ? pinv : 1.0;
+    // trailing columns: a(r, c) -= L(r, k) U(k, c) with L(r, k) = a(r, k) f,
+    // one FMA per row with f U(k, c) formed once; column k itself stays
+    // unscaled until the end (finished columns: fu = 0)
+    const double fu = (c > k) ? f * a[k] : 0.0;
 #pragma unroll
-    for (int r = k + 1; r < 16; r++) {
-      const double l = rdlane(a[r], k) * f;  // L(r,k)
-      a[r] = fma(-l, uk, a[r] * gk);
-    }
-    a[k] = mine ? piv : a[k];
-    mys = mine ? sk : mys;
-    myinv = mine ? pinv : myinv;
+    for (int r = k + 1; r < 16; r++) a[r] = fma(-rdlane(a[r], k), fu, a[r]);
+    a[k] = (act && c == k) ? piv : a[k];
   }
+  wfence();
+  // write-out: U above the diagonal with the deferred sign S[c] of its
+  // column, the pivot on it, L = column * f_c below it; with o > 0 also the
+  // rows 0..o-1 of these columns (U12 of the 32 x 32 matrix, formed before
+  // these signs were known) take their signs
   if (lane < 16) {
+    const double mys = S[o + c], myf = mys * D[o + c];
 #pragma unroll
-    for (int r = 0; r < 16; r++) M[(o + r) * LDM + o + c] = a[r];
-    S[o + c] = mys;
-    D[o + c] = myinv;
+    for (int r = 0; r < 16; r++)
+      M[(o + r) * LDM + o + c] = (r < c) ? a[r] * mys : (r == c ? a[r] : a[r] * myf);
+    if (o > 0) {
+      double u[16];
+#pragma unroll
+      for (int r = 0; r < 16; r++) u[r] = M[r * LDM + o + c];
+#pragma unroll
+      for (int r = 0; r < 16; r++) M[r * LDM + o + c] = u[r] * mys;
+    }
   }
   wfence();
 }
@@ -226,49 +241,38 @@ __device__ __forceinline__ bool wave_lu_sign32(double *M, double *S, double *D, 
   const int lane = opaque_lane();
   bool ok = true;
   lu16(M, S, D, 0, nb, ok);
-  if (lane < 16) {
-    // row 16 + lane of L21: x V = A21 row, forward over the columns
-    double x[16];
+  if (lane < 32) {
+    // one forward substitution per lane, both kinds in the same instruction
+    // stream (no divergent halves): lanes 0..15 a row of L21 = A21 V^-1 (V =
+    // A11's U with the signs S1 applied above the diagonal and the pivot
+    // times its sign on it: z[t] = (S[t] a[t] - sum_k z[k] U(k, t)) D[t]),
+    // lanes 16..31 a column of U12 = L11^-1 A12 (L11 unit lower: z[t] =
+    // a[t] - sum_k L(t, k) z[k])
+    const bool isl = lane < 16;
+    const int i = lane & 15;
+    const int sk = isl ? LDM : 1, st = isl ? 1 : LDM;  // coefficient (k, t) at k sk + t st
+    double z[16];
 #pragma unroll
-    for (int c = 0; c < 16; c++) x[c] = M[(16 + lane) * LDM + c];
+    for (int t = 0; t < 16; t++) z[t] = isl ? M[(16 + i) * LDM + t] * S[t] : M[t * LDM + 16 + i];
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
-      double acc = x[c];
+    for (int t = 0; t < 16; t++) {
+      double acc = z[t];
 #pragma unroll
-      for (int k = 0; k < c; k++) acc -= x[k] * M[k * LDM + c];
-      x[c] = acc * (S[c] * D[c]);
+      for (int k = 0; k < t; k++) acc -= z[k] * M[k * sk + t * st];
+      z[t] = isl ? acc * D[t] : acc;
     }
 #pragma unroll
-    for (int c = 0; c < 16; c++) M[(16 + lane) * LDM + c] = x[c];
-  } else if (lane < 32) {
-    // column 16 + j of U12 = L11^-1 A12 (L11 unit lower)
-    const int j = lane - 16;
-    double y[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) y[r] = M[r * LDM + 16 + j];
-#pragma unroll
-    for (int r = 1; r < 16; r++) {
-      double acc = y[r];
-#pragma unroll
-      for (int k = 0; k < r; k++) acc -= M[r * LDM + k] * y[k];
-      y[r] = acc;
+    for (int t = 0; t < 16; t++) {
+      if (isl) M[(16 + i) * LDM + t] = z[t];
+      else M[t * LDM + 16 + i] = z[t];
     }
-#pragma unroll
-    for (int r = 0; r < 16; r++) M[r * LDM + 16 + j] = y[r];
   }
   wfence();
   const d4s_t s22 = mm16(M + 16 * LDM, LDM, 1, M + 16, LDM, 1);  // L21 U12
 #pragma unroll
   for (int v = 0; v < 4; v++) M[(16 + (lane >> 4) + 4 * v) * LDM + 16 + (lane & 15)] -= s22[v];
   wfence();
-  lu16(M, S, D, 16, nb, ok);
-  // the deferred signs: U(r, c) *= S[c] above the diagonal
-#pragma unroll
-  for (int e = lane; e < PNB * PNB; e += 64) {
-    const int r = e / PNB, c = e % PNB;
-    if (r < c) M[r * LDM + c] *= S[c];
-  }
-  wfence();
+  lu16(M, S, D, 16, nb, ok);  // also applies S2 to U12
   return ok;
 }
 
