@@ -1062,6 +1062,7 @@ __device__ void slab_mul64(double *X, int RS, int nrows, int nb, const double *s
 // Gram X^T X of the slab's first nloc rows (64 columns, upper 16 x 16 tiles
 // dealt to the waves), its upper part added into the global G (64 x 64)
 __device__ void slab_gram64_add(const double *X, int RS, int nloc, int nb, double *G) {
+  if (nloc <= 0) return;  // a workgroup without rows (the one that forms T)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int u = wave; u < 10; u += 4) {
     // tile u of (0,0) (0,1) (0,2) (0,3) (1,1) (1,2) (1,3) (2,2) (2,3) (3,3)
@@ -1442,6 +1443,8 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
 
   // ---- outputs: V2 = X (R2^-1 S U^-1) = (Q2 S) U^-1 ----
+  // (a workgroup without rows -- the extra one that forms T -- skips them)
+  if (nloc > 0) {
   wg_inv64(fblk(M2, 0, 0), fblk(M2, 0, 1), fblk(M2, 1, 1), D2, M0);  // U^-1
   {
     // M4 <- M4 diag(S) M0 (upper x diag x upper), from registers after all reads
@@ -1470,6 +1473,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     __syncthreads();
   }
   slab_mul64(X, RS, nloc, nb, nullptr, M4);
+  }
   if (g == 0) {
     // top nb rows: R' = S R2 R1 on and above the diagonal (R from the
     // X_top rows), V1 = L below
@@ -1479,9 +1483,10 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     }
   }
   if (g == G - 1) {
-    // (every workgroup holds U and L: the last one, with the fewest rows,
-    // forms T while workgroup 0 writes the top rows; with one workgroup,
-    // after them)
+    // (every workgroup holds U and L: the last one -- launched without
+    // rows when the chip has a CU to spare, else the one with the fewest
+    // -- forms T while the others form V2 and workgroup 0 writes the top
+    // rows; with one workgroup, after them)
     __syncthreads();
     // T = -U (V1^T)^-1: V1^T (unit upper, identity beyond nb) into M0's
     // blocks, its inverse into M4, then the product of the two triangles
@@ -2146,6 +2151,10 @@ extern "C" int ld_panel_qr64(int ml, int nb, double *A, int lda, double *T, int 
     }
   }
   if (attr < 0 || (ncu > 0 && G > ncu)) return -1;
+  // one workgroup more than the slabs need, with no rows of its own: the
+  // last workgroup forms T (every workgroup holds U and L after the LU)
+  // while the others form and write V2, so T leaves the critical path
+  if (ncu <= 0 || G + 1 <= ncu) G++;
   PanelArgs pa;
   memset(&pa, 0, sizeof pa);
   pa.ml = ml; pa.nb = nb; pa.A = A; pa.lda = lda; pa.T = T; pa.ldt = ldt;
