@@ -488,11 +488,13 @@ static void cholqr_reconstruct_oneblock(int ml, int n, double *Aii, int lda, dou
     ld_lu_sign(n, Tq, ldt, S);
     /* Q = A (R^-1 S) */
     ld_flip_cols(n, n, Ri, n, S);
-    gemm(N_, N_, ml, n, n, 1.0, Aii, lda, LD_GEN, Ri, n, LD_UPPER, 0.0, Qii, ldq);
+    /* (R^-1 and U^-1 come from triangular solves on the identity: exact zeros
+     * below their diagonals, LD_UPPER_Z) */
+    gemm(N_, N_, ml, n, n, 1.0, Aii, lda, LD_GEN, Ri, n, LD_UPPER_Z, 0.0, Qii, ldq);
     /* V2 = (Q S)_2 U^-1 */
     ld_laset('A', n, n, 0.0, 1.0, Ui, n);
     ld_trsm('L', 'U', 'N', 'N', n, n, 1.0, Tq, ldt, Ui, n);
-    if (ml > n) gemm(N_, N_, ml - n, n, n, 1.0, Qii + n, ldq, LD_GEN, Ui, n, LD_UPPER, 0.0, Aii + n, lda);
+    if (ml > n) gemm(N_, N_, ml - n, n, n, 1.0, Qii + n, ldq, LD_GEN, Ui, n, LD_UPPER_Z, 0.0, Aii + n, lda);
     /* top block: R <- S R, V1 = L, T = -U V1^-T */
     ld_lacpy('U', n, n, G, n, Aii, lda);
     ld_flip_rows_upper(n, Aii, lda, S);

@@ -431,7 +431,7 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
                                                          const double *__restrict__ A, int lda,
                                                          const double *__restrict__ B, int ldb,
                                                          double *__restrict__ C, int ldc, int kchunk,
-                                                         double *__restrict__ P) {
+                                                         double *__restrict__ P, int kupper) {
   extern __shared__ __attribute__((aligned(16))) double g2s[];
   double *As = g2s;                   // [2][G2_ASTAGE]
   double *Bs = g2s + 2 * G2_ASTAGE;   // [2][G2_BSTAGE]
@@ -451,8 +451,10 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
   const int wm = wave & 1, wn = wave >> 1;
   const int m0 = tx * G2_BM, n0 = ty * G2_BN;
   const int kbeg = blockIdx.z * kchunk;
-  const int kend = min(K, kbeg + kchunk);
-  const int nsteps = (kend - kbeg) / G2_BK;
+  // an upper triangular B with zeros below its diagonal: rows of B past the
+  // tile's last column add nothing
+  const int kend = min(kupper ? min(K, n0 + G2_BN) : K, kbeg + kchunk);
+  const int nsteps = max(0, (kend - kbeg) / G2_BK);
 
   // this wave's four load instructions per operand and stage; load jj of a
   // k-contiguous operand fills columns 8 jj .. 8 jj + 7
@@ -603,10 +605,19 @@ static void gemm_tiles(const ld_gemm_t *g);
 static bool g2_eligible(const ld_gemm_t *g) {
   // (products with op(A) = A^T run on it only with gemm_v2 = 2: in the
   // factorization they measured no faster than on the tiled kernel)
-  return (!g->ta || ld_tune()->gemm_v2 >= 2) && g->sa == LD_GEN && g->sb == LD_GEN && !g->tb &&
+  return (!g->ta || ld_tune()->gemm_v2 >= 2) && g->sa == LD_GEN &&
+         (g->sb == LD_GEN || g->sb == LD_UPPER_Z) && !g->tb &&
          !g->c_upper && g->zero_rows == 0 && !ld_tune()->gemm_lowp && ld_tune()->gemm_v2 &&
          ((uintptr_t)g->A & 15) == 0 && ((uintptr_t)g->B & 15) == 0 && (g->lda & 1) == 0 &&
          (g->ldb & 1) == 0;
+}
+
+// inner-product length summed over the columns of a product with an
+// upper triangular B whose K range is capped per 128-column tile
+static double flops_upper_cols(int Ni, int Ki) {
+  double s = 0.0;
+  for (int n0 = 0; n0 < Ni; n0 += G2_BN) s += (double)G2_BN * (n0 + G2_BN < Ki ? n0 + G2_BN : Ki);
+  return s;
 }
 
 static void gemm2_general(const ld_gemm_t *g) {
@@ -657,13 +668,14 @@ static void gemm2_general(const ld_gemm_t *g) {
     }
     const int ph = ld_prof_begin(g->ta ? "gemm2_f64_kernel<true>" : "gemm2_f64_kernel<false>");
     dim3 grid(Mi / G2_BM, Ni / G2_BN, splits);
+    const int kup = (g->sb == LD_UPPER_Z) ? 1 : 0;
     if (g->ta)
       hipLaunchKernelGGL(gemm2_f64_kernel<true>, grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha, g->beta,
-                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P);
+                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P, kup);
     else
       hipLaunchKernelGGL(gemm2_f64_kernel<false>, grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha, g->beta,
-                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P);
-    ld_prof_end(ph, 2.0 * Mi * Ni * Ki,
+                         g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P, kup);
+    ld_prof_end(ph, kup ? 2.0 * Mi * flops_upper_cols(Ni, Ki) : 2.0 * Mi * Ni * Ki,
                 8.0 * ((double)Mi * Ki + (double)Ki * Ni + (splits > 1 ? (double)splits : 2.0) * Mi * Ni));
     if (splits > 1) {
       const size_t tot = (size_t)Mi * Ni;
@@ -674,9 +686,11 @@ static void gemm2_general(const ld_gemm_t *g) {
       ld_ws_release(mark);
     }
   }
-  // the K tail onto the interior (beta = 1 for every row)
+  // the K tail onto the interior (beta = 1 for every row); with LD_UPPER_Z
+  // the strips read B's stored zeros as a general operand
   if (Ki < K) {
     ld_gemm_t h = *g;
+    if (h.sb == LD_UPPER_Z) h.sb = LD_GEN;
     h.m = Mi; h.n = Ni; h.k = K - Ki; h.beta = 1.0; h.zero_rows = 0;
     h.A = g->ta ? g->A + Ki : g->A + (size_t)Ki * g->lda;
     h.B = g->B + Ki;
@@ -685,6 +699,7 @@ static void gemm2_general(const ld_gemm_t *g) {
   // right strip (all rows), bottom strip
   if (Ni < N) {
     ld_gemm_t h = *g;
+    if (h.sb == LD_UPPER_Z) h.sb = LD_GEN;
     h.n = N - Ni;
     h.B = g->B + (size_t)Ni * g->ldb;
     h.C = g->C + (size_t)Ni * g->ldc;
@@ -692,6 +707,7 @@ static void gemm2_general(const ld_gemm_t *g) {
   }
   if (Mi < M) {
     ld_gemm_t h = *g;
+    if (h.sb == LD_UPPER_Z) h.sb = LD_UPPER;
     h.m = M - Mi; h.n = Ni;
     h.A = g->ta ? g->A + (size_t)Mi * g->lda : g->A + Mi;
     h.C = g->C + Mi;
@@ -703,7 +719,8 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
   if (ld_failed()) return;  // an earlier allocation of this call failed
   if (g->m <= 0 || g->n <= 0) return;
   if (g->k > 0 && g->alpha != 0.0 && ld_tune()->gemm_v2 && !ld_tune()->gemm_lowp && !g->tb &&
-      g->sb == LD_GEN && !g->c_upper && (g->sa == LD_GEN || g->sa == LD_VUNIT)) {
+      (g->sb == LD_GEN || (g->sb == LD_UPPER_Z && g->sa == LD_GEN)) && !g->c_upper &&
+      (g->sa == LD_GEN || g->sa == LD_VUNIT)) {
     // the general part of the product (all of it, or what is left of a
     // Householder V once its unit lower top block is split off: C's first k
     // rows when op(A) = V, the first m rows of K when op(A) = V^T)
@@ -737,8 +754,13 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
   gemm_tiles(g);
 }
 
-static void gemm_tiles(const ld_gemm_t *g) {
-  if (g->m <= 0 || g->n <= 0) return;
+static void gemm_tiles(const ld_gemm_t *g0) {
+  if (g0->m <= 0 || g0->n <= 0) return;
+  // the tiled kernel knows LD_UPPER_Z as LD_UPPER (it masks the lower part)
+  ld_gemm_t gz = *g0;
+  if (gz.sa == LD_UPPER_Z) gz.sa = LD_UPPER;
+  if (gz.sb == LD_UPPER_Z) gz.sb = LD_UPPER;
+  const ld_gemm_t *g = &gz;
   hipStream_t st = (hipStream_t)ld_stream();
   if (g->k <= 0 || g->alpha == 0.0) {
     // C = beta C (rows < zero_rows -> 0)

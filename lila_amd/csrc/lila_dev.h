@@ -19,7 +19,9 @@ extern "C" {
 /* structure of a stored operand (applied to stored row r, column c before
  * any transpose): general, unit lower trapezoidal (Householder V: 0 above,
  * 1 on the diagonal), or upper triangular (T, R: 0 below the diagonal). */
-enum { LD_GEN = 0, LD_VUNIT = 1, LD_UPPER = 2, LD_LOWER = 3, LD_UUNIT = 4 };
+/* LD_UPPER_Z: upper triangular with exact zeros stored below the diagonal
+ * (read as a general matrix; the zeros only shorten the K range) */
+enum { LD_GEN = 0, LD_VUNIT = 1, LD_UPPER = 2, LD_LOWER = 3, LD_UUNIT = 4, LD_UPPER_Z = 5 };
 
 typedef struct {
     int m, n, k;

@@ -7,16 +7,41 @@
 // leaf is applied as a GEMM with its explicitly inverted diagonal block
 // (all leaves inverted up front in one launch, one workgroup per block).
 // Cholesky is recursive in the same way (TRSM + SYRK-form GEMM), with a
-// single-workgroup unblocked factorization of each 64 x 64 leaf.
+// single-workgroup factorization of each 64 x 64 leaf (the one-wave 32 x 32
+// routines of small32.h, the column loop when a pivot fails).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 #include "lila_dev.h"
+#include "small32.h"
 
 extern "C" int *ld_device_flags(void);
 
 namespace {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+using small32::LDM;
+constexpr int B32 = 32 * LDM;  // one 32 x 32 block (row-major, ld LDM)
+constexpr int WG = 256;        // threads of the 64 x 64 leaf kernels
+
+// block (bi, bj), bi <= bj, of an upper 64 x 64 matrix stored as 3 blocks
+__device__ __forceinline__ double *ub(double *M, int bi, int bj) { return M + (bi + bj) * B32; }
+__device__ __forceinline__ double &uel(double *M, int r, int c) {
+  return M[((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)];
+}
+
+// X = M^-1 for the upper 64 x 64 M in 3-block storage (lower parts zero),
+// Dg = 1 / diag(M): the two diagonal 32 x 32 inverses by waves 0 and 1 at
+// once, then X12 = -(X11 M12) X22 on MFMA (Tm: one free block)
+__device__ void inv_upper64(double *M, const double *Dg, double *X, double *Tm) {
+  const int wave = threadIdx.x >> 6;
+  if (wave == 0) small32::inv_upper32(ub(M, 0, 0), Dg, ub(X, 0, 0));
+  else if (wave == 1) small32::inv_upper32(ub(M, 1, 1), Dg + 32, ub(X, 1, 1));
+  __syncthreads();
+  small32::wg_mm32(ub(X, 0, 0), LDM, 1, false, ub(M, 0, 1), LDM, 1, Tm, 1.0, false);
+  small32::wg_mm32(Tm, LDM, 1, false, ub(X, 1, 1), LDM, 1, ub(X, 0, 1), -1.0, false);
+}
 
 constexpr int TB = 64;  // diagonal block of TRSM / Cholesky leaves
 
@@ -35,46 +60,41 @@ inline void gemm(int ta, int tb, int m, int n, int k, double alpha, const double
 }
 
 // inverse of each TB x TB diagonal block of the stored triangle (uplo,
-// unit) -> D[b] (TB x TB, zero outside the triangle).  One workgroup of 64
-// threads per block; thread c builds column c by substitution.
-__global__ void trtri_blocks_kernel(int nn, int upper, int unit, const double *__restrict__ A,
-                                    int lda, double *__restrict__ D) {
-  __shared__ double L[TB][TB + 1];
-  __shared__ double X[TB][TB + 1];
-  const int b = blockIdx.x, c = threadIdx.x;
-  const int o = b * TB;
-  const int bs = min(TB, nn - o);
-  for (int r = 0; r < TB; r++) {
+// unit) -> D[b] (TB x TB, zero outside the triangle), one workgroup of 256
+// threads per block on the one-wave 32 x 32 routines and MFMA (a lower
+// triangle is inverted as its transpose)
+__global__ __launch_bounds__(WG) void trtri64_kernel(int nn, int upper, int unit,
+                                                     const double *__restrict__ A, int lda,
+                                                     double *__restrict__ D) {
+  __shared__ double M[3 * B32], X[3 * B32], Tm[B32], Dg[64];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int o = b * TB, bs = min(TB, nn - o);
+  for (int t = tid; t < 64 * 64; t += WG) {
+    const int r = t & 63, c = t >> 6;  // M(r, c), upper
+    if (r >= 32 && c < 32) continue;
     double v = 0.0;
     if (r < bs && c < bs) {
-      bool in = upper ? (r <= c) : (r >= c);
-      if (in) v = A[(size_t)(o + r) + (size_t)(o + c) * lda];
-      if (r == c && unit) v = 1.0;
+      if (r < c) v = upper ? A[(size_t)(o + r) + (size_t)(o + c) * lda] : A[(size_t)(o + c) + (size_t)(o + r) * lda];
+      else if (r == c) v = unit ? 1.0 : A[(size_t)(o + r) + (size_t)(o + r) * lda];
+    } else if (r == c) {
+      v = 1.0;
     }
-    L[r][c] = v;
-    X[r][c] = 0.0;
+    uel(M, r, c) = v;
   }
   __syncthreads();
-  if (c < bs) {
-    if (!upper) {
-      X[c][c] = 1.0 / L[c][c];
-      for (int r = c + 1; r < bs; r++) {
-        double s = 0.0;
-        for (int p = c; p < r; p++) s += L[r][p] * X[p][c];
-        X[r][c] = -s / L[r][r];
-      }
-    } else {
-      X[c][c] = 1.0 / L[c][c];
-      for (int r = c - 1; r >= 0; r--) {
-        double s = 0.0;
-        for (int p = r + 1; p <= c; p++) s += L[r][p] * X[p][c];
-        X[r][c] = -s / L[r][r];
-      }
-    }
-  }
+  if (tid < 64) Dg[tid] = 1.0 / uel(M, tid, tid);
   __syncthreads();
+  inv_upper64(M, Dg, X, Tm);
   double *Db = D + (size_t)b * TB * TB;
-  for (int r = 0; r < TB; r++) Db[r + (size_t)c * TB] = X[r][c];
+  for (int t = tid; t < 64 * 64; t += WG) {
+    const int r = t & 63, c = t >> 6;  // D(r, c), column-major
+    double v = 0.0;
+    if (r < bs && c < bs) {
+      if (upper && r <= c) v = uel(X, r, c);
+      else if (!upper && r >= c) v = uel(X, c, r);
+    }
+    Db[r + (size_t)c * TB] = v;
+  }
 }
 
 __global__ void scale_kernel(int m, int n, double alpha, double *B, int ldb) {
@@ -153,19 +173,71 @@ void trsm_rec(const TrsmCtx &t, int o, int len) {
 
 // unblocked upper Cholesky of one leaf: U = chol(A_bb(upper) - Sub), Sub
 // optional (bs x bs, ld bs); flags[1] gets the first failing global column.
-__global__ void potf2_kernel(int bs, double *A, int lda, const double *Sub, int col0, int *flags) {
+__device__ void potf2_body(int bs, double *A, int lda, const double *Sub, int col0, int *flags,
+                           double (&U)[TB][TB + 1], int &bad);
+
+// upper Cholesky of a leaf of at most 64 columns by 256 threads: R11 =
+// chol(A11) and R22 = chol(A22 - R12^T R12) by the one-wave 32 x 32 routine
+// (waves 1-3 idle meanwhile), R12 = R11^-T A12 on MFMA.  A pivot that is not
+// positive sends the leaf to the column-by-column loop (threads 0..63), which
+// also finds the first failing column for the LAPACK-style info.
+__global__ __launch_bounds__(WG) void potf2_64_kernel(int bs, double *A, int lda, int col0, int *flags) {
+  __shared__ double M[3 * B32], Sc[B32], Dg[64];
+  __shared__ int okf;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  for (int t = tid; t < 64 * 64; t += WG) {
+    const int r = t & 63, c = t >> 6;
+    if (r >= 32 && c < 32) continue;
+    uel(M, r, c) = (r < bs && c < bs) ? (r <= c ? A[(size_t)r + (size_t)c * lda] : 0.0) : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const bool ok = small32::wave_chol32(ub(M, 0, 0), Dg, bs < 32 ? bs : 32, false);
+    if (tid == 0) okf = ok ? 1 : 0;
+  }
+  __syncthreads();
+  if (okf) {
+    if (wave == 0) small32::inv_upper32(ub(M, 0, 0), Dg, Sc);
+    __syncthreads();
+    // R12 = R11^-T A12, then A22 -= R12^T R12
+    small32::wg_mm32(Sc, 1, LDM, false, ub(M, 0, 1), LDM, 1, ub(M, 0, 1), 1.0, false);
+    small32::wg_mm32(ub(M, 0, 1), 1, LDM, false, ub(M, 0, 1), LDM, 1, ub(M, 1, 1), -1.0, true);
+    if (wave == 0) {
+      const bool ok = small32::wave_chol32(ub(M, 1, 1), Dg + 32, bs > 32 ? bs - 32 : 0, false);
+      if (tid == 0) okf = ok ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  if (okf) {
+    for (int t = tid; t < 64 * 64; t += WG) {
+      const int r = t & 63, c = t >> 6;
+      if (r <= c && c < bs) A[(size_t)r + (size_t)c * lda] = uel(M, r, c);
+    }
+    return;
+  }
+  // the column loop on the leaf as it came in (A is still unchanged); all
+  // threads take part in its barriers, threads 0..63 do the work
   __shared__ double U[TB][TB + 1];
   __shared__ int bad;
+  potf2_body(bs, A, lda, nullptr, col0, flags, U, bad);
+}
+
+__device__ void potf2_body(int bs, double *A, int lda, const double *Sub, int col0, int *flags,
+                           double (&U)[TB][TB + 1], int &bad) {
+  // thread c < TB keeps column c; threads past TB (a larger workgroup) only
+  // take part in the barriers
   const int c = threadIdx.x;
+  const bool mine = c < TB;
   if (c == 0) bad = 0;
-  for (int r = 0; r < TB; r++) {
-    double v = 0.0;
-    if (r < bs && c < bs && r <= c) {
-      v = A[(size_t)r + (size_t)c * lda];
-      if (Sub) v -= Sub[r + (size_t)c * bs];
+  if (mine)
+    for (int r = 0; r < TB; r++) {
+      double v = 0.0;
+      if (r < bs && c < bs && r <= c) {
+        v = A[(size_t)r + (size_t)c * lda];
+        if (Sub) v -= Sub[r + (size_t)c * bs];
+      }
+      U[r][c] = v;
     }
-    U[r][c] = v;
-  }
   __syncthreads();
   for (int j = 0; j < bs; j++) {
     // U(j,j) and row j of U
@@ -178,21 +250,21 @@ __global__ void potf2_kernel(int bs, double *A, int lda, const double *Sub, int 
       U[j][j] = (d > 0.0) ? sqrt(d) : d;
     }
     __syncthreads();
-    if (c > j && c < bs) U[j][c] /= U[j][j];
+    if (mine && c > j && c < bs) U[j][c] /= U[j][j];
     __syncthreads();
-    if (c > j && c < bs)
+    if (mine && c > j && c < bs)
       for (int r = j + 1; r <= c; r++) U[r][c] -= U[j][r] * U[j][c];
     __syncthreads();
   }
-  for (int r = 0; r < bs; r++)
-    if (c < bs && r <= c) A[(size_t)r + (size_t)c * lda] = U[r][c];
+  if (mine)
+    for (int r = 0; r < bs; r++)
+      if (c < bs && r <= c) A[(size_t)r + (size_t)c * lda] = U[r][c];
 }
 
 void potrf_rec(int n, double *A, int lda, int col0, const double *Dscratch_unused) {
   (void)Dscratch_unused;
   if (n <= TB) {
-    hipLaunchKernelGGL(potf2_kernel, dim3(1), dim3(TB), 0, S(), n, A, lda, (const double *)nullptr,
-                       col0, ld_device_flags());
+    hipLaunchKernelGGL(potf2_64_kernel, dim3(1), dim3(WG), 0, S(), n, A, lda, col0, ld_device_flags());
     return;
   }
   int nblk = (n + TB - 1) / TB;
@@ -267,6 +339,45 @@ __global__ void lu_sign_kernel(int n, int nrow, double *T, int ldt, double *Sg, 
       T[(size_t)r + (size_t)c * ldt] -= T[(size_t)r + (size_t)k * ldt] * T[(size_t)k + (size_t)c * ldt];
     }
     __syncthreads();
+  }
+}
+
+// rule 0 on an mr x nc panel (nc <= 32) by 256 threads: the top 32 x 32
+// (identity past the panel) by the one-wave sign-chosen LU -- the same
+// pivots, signs and factors as the column loop above -- and the rows below
+// it, whose L rows solve x U = a S (U with its signs, the pivots on its
+// diagonal), one row per thread
+__global__ __launch_bounds__(WG) void lu_sign32_leaf_kernel(int nc, int mr, double *T, int ldt, double *Sg) {
+  __shared__ double M[B32], Sv[32], Dv[32];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  for (int t = tid; t < 32 * 32; t += WG) {
+    const int r = t & 31, c = t >> 5;
+    M[r * LDM + c] = (r < mr && c < nc) ? T[(size_t)r + (size_t)c * ldt] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  if (wave == 0) (void)small32::wave_lu_sign32(M, Sv, Dv, nc);
+  __syncthreads();
+  for (int t = tid; t < 32 * 32; t += WG) {
+    const int r = t & 31, c = t >> 5;
+    if (r < mr && c < nc) T[(size_t)r + (size_t)c * ldt] = M[r * LDM + c];
+  }
+  if (tid < nc) Sg[tid] = Sv[tid];
+  // rows below the top 32: one row per thread, x U = a S by forward
+  // substitution against U in LDS (U(c, c) the pivot, 1 / pivot in Dv)
+  for (int r = 32 + tid; r < mr; r += WG) {
+    double x[32];
+#pragma unroll
+    for (int c = 0; c < 32; c++) x[c] = (c < nc) ? T[(size_t)r + (size_t)c * ldt] * Sv[c] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 32; c++) {
+      double acc = x[c];
+#pragma unroll
+      for (int k = 0; k < c; k++) acc -= x[k] * M[k * LDM + c];
+      x[c] = acc * Dv[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 32; c++)
+      if (c < nc) T[(size_t)r + (size_t)c * ldt] = x[c];
   }
 }
 
@@ -380,7 +491,7 @@ void ld_trsm(char side, char uplo, char trans, char diag, int m, int n, double a
   const int nblk = (nn + TB - 1) / TB;
   size_t mark = ld_ws_mark();
   double *D = (double *)ld_ws_alloc(sizeof(double) * (size_t)nblk * TB * TB);
-  hipLaunchKernelGGL(trtri_blocks_kernel, dim3(nblk), dim3(TB), 0, S(), nn, uplo == 'U' ? 1 : 0,
+  hipLaunchKernelGGL(trtri64_kernel, dim3(nblk), dim3(WG), 0, S(), nn, uplo == 'U' ? 1 : 0,
                      diag == 'U' ? 1 : 0, A, lda, D);
   TrsmCtx t;
   t.side = side;
@@ -475,7 +586,10 @@ void ld_n2t(int n, const double *tau, double *T, int ldt) {
 static void lu_sign_panel(int mr, int nc, double *A, int lda, double *Sg, int rule) {
   if (nc <= 0) return;
   if (nc <= 32) {
-    hipLaunchKernelGGL(lu_sign_kernel, dim3(1), dim3(1024), 0, S(), nc, mr, A, lda, Sg, rule);
+    if (rule == 0)
+      hipLaunchKernelGGL(lu_sign32_leaf_kernel, dim3(1), dim3(WG), 0, S(), nc, mr, A, lda, Sg);
+    else
+      hipLaunchKernelGGL(lu_sign_kernel, dim3(1), dim3(1024), 0, S(), nc, mr, A, lda, Sg, rule);
     return;
   }
   const int n1 = (nc / 2 + 31) / 32 * 32, n2 = nc - n1;

@@ -63,6 +63,7 @@ using small32::rdlane;
 using small32::wave_chol32;
 using small32::wave_lu_sign32;
 using small32::wfence;
+using small32::wg_mm32;
 // persistent global scratch layout (doubles)
 constexpr int OFF_RED = 0;                   // 3 x SLOT reduction vectors
 constexpr int OFF_V1 = OFF_RED + 3 * SLOT;   // V1 staging (PNB*PNB)
@@ -904,30 +905,6 @@ __device__ __forceinline__ double lul(const double *M, int r, int c) {
   return M[(2 * (r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)];
 }
 
-// C = alpha A B (+ C when acc) for 32 x 32 blocks, all four waves (wave w
-// the 16 x 16 output tile (w & 1, w >> 1)): A(i, k) at A[i ars + k acs] (0
-// for i > k when amask: an upper factor whose lower part holds something
-// else), B(k, j) at B[k brs + j bcs], C row-major with ld LDM.  C may alias
-// A or B: every read precedes the first write.
-__device__ void wg_mm32(const double *A, int ars, int acs, bool amask, const double *B, int brs, int bcs,
-                        double *C, double alpha, bool acc) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ti = wave & 1, tj = wave >> 1, i = ti * 16 + (lane & 15), j = tj * 16 + (lane & 15);
-  d4_t r = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int ks = 0; ks < 8; ks++) {
-    const int k = 4 * ks + (lane >> 4);
-    const double av = (amask && i > k) ? 0.0 : A[i * ars + k * acs];
-    r = __builtin_amdgcn_mfma_f64_16x16x4f64(av, B[k * brs + j * bcs], r, 0, 0, 0);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int v = 0; v < 4; v++) {
-    double *c = C + (ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15);
-    *c = acc ? *c + alpha * r[v] : alpha * r[v];
-  }
-  __syncthreads();
-}
 
 // upper Cholesky of the 64 x 64 G in M (3-block storage, upper part read,
 // identity beyond nb) in place; 1 / diag(R) into D; Sc a free block.

@@ -131,7 +131,7 @@ __device__ __forceinline__ void inv16(const double *R, const double *D, int o, d
 // pivots are those of the column-by-column algorithm.  Returns false if a
 // pivot is not positive or R's first nb diagonal entries spread by more
 // than 1e5.
-__device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb) {
+__device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb, bool spread = true) {
   const int lane = opaque_lane();
   bool ok = true;
   double dmin = 1e308, dmax = 0.0;
@@ -150,7 +150,7 @@ __device__ __forceinline__ bool wave_chol32(double *M, double *D, int nb) {
 #pragma unroll
   for (int e = lane; e < 256; e += 64) M[(16 + e / 16) * LDM + e % 16] = 0.0;
   wfence();
-  return ok && dmin > 1e-5 * dmax;
+  return ok && (!spread || dmin > 1e-5 * dmax);
 }
 
 // 1 / x for the pivots of the sign-chosen LU (|x| = |t| + 1 >= 1, finite):
@@ -325,6 +325,31 @@ __device__ __forceinline__ void inv_upper32(const double *R, const double *D, do
 #pragma unroll
   for (int v = 0; v < 4; v++) X[((lane >> 4) + 4 * v) * LDM + 16 + c] = -u[v];
   wfence();
+}
+
+// C = alpha A B (+ C when acc) for 32 x 32 blocks, all four waves of a 256-thread workgroup (wave w
+// the 16 x 16 output tile (w & 1, w >> 1)): A(i, k) at A[i ars + k acs] (0
+// for i > k when amask: an upper factor whose lower part holds something
+// else), B(k, j) at B[k brs + j bcs], C row-major with ld LDM.  C may alias
+// A or B: every read precedes the first write.
+__device__ __forceinline__ void wg_mm32(const double *A, int ars, int acs, bool amask, const double *B, int brs, int bcs,
+                        double *C, double alpha, bool acc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ti = wave & 1, tj = wave >> 1, i = ti * 16 + (lane & 15), j = tj * 16 + (lane & 15);
+  d4s_t r = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < 8; ks++) {
+    const int k = 4 * ks + (lane >> 4);
+    const double av = (amask && i > k) ? 0.0 : A[i * ars + k * acs];
+    r = __builtin_amdgcn_mfma_f64_16x16x4f64(av, B[k * brs + j * bcs], r, 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double *c = C + (ti * 16 + (lane >> 4) + 4 * v) * LDM + tj * 16 + (lane & 15);
+    *c = acc ? *c + alpha * r[v] : alpha * r[v];
+  }
+  __syncthreads();
 }
 
 }  // namespace small32
