@@ -464,8 +464,8 @@ static void cholqr(int ml, int n, double *Aii, int lda, double *Qii, int ldq) {
 /* Cholesky-QR and the Householder reconstruction of one block (the whole
  * n columns inside one T block, so the reconstruction has no earlier blocks:
  * lila-c/lila_dgeqrf_w03_mt_hr.c:18-29 with a single lila_ormhr2_w03_hr
- * step), one FP64 Gram pass, arranged so that the ml x n data is read or
- * written five times instead of about fourteen:
+ * step), arranged so that the ml x n data is read or written five times
+ * (one Gram pass) instead of about fourteen:
  *   G = A^T A = R^T R (A read in place, no copy of A into Q first);
  *   Q_top = A_top R^-1 (n x n) and its sign-chosen LU: Q_top S - I = L U;
  *   Q = A (R^-1 S), one GEMM with an upper triangular right operand, which
@@ -473,30 +473,55 @@ static void cholqr(int ml, int n, double *Aii, int lda, double *Qii, int ldq) {
  *   V2 = (Q S)_2 U^-1 into A below the top block, one more such GEMM;
  *   R <- S R, V1 = L, T = -U V1^-T on the n x n top block.
  * The same V, R, T and Q as the TRSM-and-copy sequence, to rounding (the
- * triangular solves run as products with explicit inverses). */
-static void cholqr_reconstruct_oneblock(int ml, int n, double *Aii, int lda, double *Tq, int ldt,
-                                        double *Qii, int ldq, double *S) {
+ * triangular solves run as products with explicit inverses).
+ * Cholesky-QR2 (prec 1, 2, 3: the first Gram in FP64, FP32 MFMA or split
+ * BF16, the second in FP64) forms W = A R1^-1 in a workspace first and runs
+ * the same steps on W with its factor R2, R = R2 R1: seven passes. */
+static void cholqr_reconstruct_oneblock(int prec, int ml, int n, double *Aii, int lda, double *Tq,
+                                        int ldt, double *Qii, int ldq, double *S) {
     size_t mark = ld_ws_mark();
     double *G = wsd((size_t)n * n), *Ri = wsd((size_t)n * n), *Ui = wsd((size_t)n * n);
-    dv_gram(0, ml, n, Aii, lda, G, n);
+    const double *X = Aii;  /* the matrix whose Cholesky-QR gives Q */
+    int ldx = lda;
+    double *R1 = NULL;
+    if (prec != 0) {
+        /* first pass: W = A R1^-1 (R1^-1 from a triangular solve on the
+         * identity: exact zeros below its diagonal) */
+        R1 = wsd((size_t)n * n);
+        const int ldw = ml + (ml & 1);  /* even: the direct-to-LDS GEMM takes it */
+        double *W = wsd((size_t)ldw * n);
+        dv_gram(prec, ml, n, Aii, lda, R1, n);
+        ld_potrf_u(n, R1, n);
+        ld_laset('A', n, n, 0.0, 1.0, Ri, n);
+        ld_trsm('L', 'U', 'N', 'N', n, n, 1.0, R1, n, Ri, n);
+        gemm(N_, N_, ml, n, n, 1.0, Aii, lda, LD_GEN, Ri, n, LD_UPPER_Z, 0.0, W, ldw);
+        X = W;
+        ldx = ldw;
+    }
+    dv_gram(prec != 0 ? 1 : 0, ml, n, X, ldx, G, n);
     ld_potrf_u(n, G, n);
     /* R^-1 (upper) */
     ld_laset('A', n, n, 0.0, 1.0, Ri, n);
     ld_trsm('L', 'U', 'N', 'N', n, n, 1.0, G, n, Ri, n);
-    /* Q_top = A_top R^-1 into the T block, its sign-chosen LU there */
-    gemm(N_, N_, n, n, n, 1.0, Aii, lda, LD_GEN, Ri, n, LD_UPPER, 0.0, Tq, ldt);
+    /* Q_top = X_top R^-1 into the T block, its sign-chosen LU there */
+    gemm(N_, N_, n, n, n, 1.0, X, ldx, LD_GEN, Ri, n, LD_UPPER, 0.0, Tq, ldt);
     ld_lu_sign(n, Tq, ldt, S);
-    /* Q = A (R^-1 S) */
+    /* Q = X (R^-1 S) */
     ld_flip_cols(n, n, Ri, n, S);
     /* (R^-1 and U^-1 come from triangular solves on the identity: exact zeros
      * below their diagonals, LD_UPPER_Z) */
-    gemm(N_, N_, ml, n, n, 1.0, Aii, lda, LD_GEN, Ri, n, LD_UPPER_Z, 0.0, Qii, ldq);
+    gemm(N_, N_, ml, n, n, 1.0, X, ldx, LD_GEN, Ri, n, LD_UPPER_Z, 0.0, Qii, ldq);
     /* V2 = (Q S)_2 U^-1 */
     ld_laset('A', n, n, 0.0, 1.0, Ui, n);
     ld_trsm('L', 'U', 'N', 'N', n, n, 1.0, Tq, ldt, Ui, n);
     if (ml > n) gemm(N_, N_, ml - n, n, n, 1.0, Qii + n, ldq, LD_GEN, Ui, n, LD_UPPER_Z, 0.0, Aii + n, lda);
-    /* top block: R <- S R, V1 = L, T = -U V1^-T */
-    ld_lacpy('U', n, n, G, n, Aii, lda);
+    /* top block: R <- S R (R = R2 R1 after two passes), V1 = L, T = -U V1^-T */
+    if (R1) {
+        gemm(N_, N_, n, n, n, 1.0, G, n, LD_UPPER, R1, n, LD_UPPER, 0.0, Ri, n);
+        ld_lacpy('U', n, n, Ri, n, Aii, lda);
+    } else {
+        ld_lacpy('U', n, n, G, n, Aii, lda);
+    }
     ld_flip_rows_upper(n, Aii, lda, S);
     ld_move_lower(n, Tq, ldt, Aii, lda);
     ld_trsm('R', 'L', 'T', 'U', n, n, -1.0, Aii, lda, Tq, ldt);
@@ -507,11 +532,12 @@ static void cholqr_reconstruct_oneblock(int ml, int n, double *Aii, int lda, dou
 int dv_dgeqrf_w03_mt_hr(int m, int n, int i, int mt, double *A, int lda, double *T, int ldt, double *Q,
                         int ldq) {
     int ml = m - i;
-    if (ld_tune()->gram_prec == 0 && i % mt + n <= mt && ml >= n) {
+    if (i % mt + n <= mt && ml >= n) {
         size_t mark = ld_ws_mark();
         double *Sd = wsd((size_t)(i + n));
-        cholqr_reconstruct_oneblock(ml, n, A + i + (size_t)i * lda, lda, T + i % mt + (size_t)i * ldt,
-                                    ldt, Q + i + (size_t)i * ldq, ldq, Sd + i);
+        cholqr_reconstruct_oneblock(ld_tune()->gram_prec, ml, n, A + i + (size_t)i * lda, lda,
+                                    T + i % mt + (size_t)i * ldt, ldt, Q + i + (size_t)i * ldq, ldq,
+                                    Sd + i);
         ld_ws_release(mark);
         return 0;
     }
@@ -529,10 +555,10 @@ int dv_dgeqr2_w03_hr(int m, int n, int i, int mt, double *A, int lda, double *T,
     int ml = m - i;
     double *Aii = A + i + (size_t)i * lda, *Qii = Q + i + (size_t)i * ldq;
     double *Tki = T + i % mt + (size_t)i * ldt;
-    if (ld_tune()->gram_prec == 0 && ml >= n) {
+    if (ml >= n) {
         size_t mark = ld_ws_mark();
         double *S = wsd((size_t)n);
-        cholqr_reconstruct_oneblock(ml, n, Aii, lda, Tki, ldt, Qii, ldq, S);
+        cholqr_reconstruct_oneblock(ld_tune()->gram_prec, ml, n, Aii, lda, Tki, ldt, Qii, ldq, S);
         ld_ws_release(mark);
         return 0;
     }
