@@ -91,6 +91,7 @@ struct PanelArgs {
   double *scr;        // persistent global scratch (SCRATCH doubles)
   unsigned *cnt;      // [1] exits
   unsigned *bar;      // grid barrier counters (in the persistent scratch)
+  double *scr_next;   // 64-wide panel: the other parity's scratch set (zeroed here for the next launch)
   int *err;           // [0] barrier time-out, [2] a streamed panel declined
   int decline;        // streamed panel: leave a panel that fails the acceptance
                       // tests untouched and raise err[2] (no column loop)
@@ -1134,26 +1135,17 @@ __device__ bool wg_lu64(double *M, double *S, double *D, double *Sc, const doubl
 
 // last workgroup out re-zeroes the 64-wide accumulation buffers and the
 // counters
-__device__ void exit_protocol64(const PanelArgs &a) {
-  __syncthreads();
-  __shared__ unsigned last64;
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    const unsigned old = __hip_atomic_fetch_add(&a.cnt[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    last64 = (old == (unsigned)a.G - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (last64) {
-    for (int t = tid; t < 3 * SLOT; t += PNT) st_coh(&a.scr[OFF_RED + t], 0.0);
-    for (int t = tid; t < SCRATCH64 - OFF64_G1; t += PNT) st_coh(&a.scr[OFF64_G1 + t], 0.0);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (tid == 0) {
-      reset_barrier(a.bar);
-      __hip_atomic_store(&a.cnt[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+// The 64-wide panel keeps no exit protocol: its launches alternate between
+// two scratch sets (accumulators, reduction slots, barrier counters), and
+// each launch clears the set the next one will use -- the previous launch
+// on the stream, which used it, has finished -- from a workgroup that has
+// time for it, so no workgroup waits on an exit counter at the end.
+__device__ void clear_set64(double *scr) {
+  for (int t = threadIdx.x; t < 3 * SLOT; t += PNT) st_coh(&scr[OFF_RED + t], 0.0);
+  for (int t = threadIdx.x; t < SCRATCH64 - OFF64_G1; t += PNT) st_coh(&scr[OFF64_G1 + t], 0.0);
+  if (threadIdx.x == 0) reset_barrier((unsigned *)(scr + OFF_BAR));
 }
+
 
 // V1(r0 + lr, k) of the slab after the left columns' column loop: unit
 // lower in the panel's rows 0..31 (R11 lies above), as stored below
@@ -1331,6 +1323,7 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   unsigned nbar = 0;
   bool near2 = false;  // pass 2's Gram within NEAR_ID of I: no Cholesky chain
 
+  if (g == G - 1) clear_set64(a.scr_next);  // (the last workgroup has the fewest rows)
   // slab and, at rows rp .., the panel's top 64 rows (see panel_cqr_kernel)
   {
     // wave w loads columns w, w + 4, ..., lanes down the rows; all sixteen
@@ -1415,7 +1408,6 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   if (!ok) {
     load_slab(a, X, r0, nloc);
     hh_fallback64(a, (int)(M2 - smem), (int)(M0 - smem), r0, nloc, nbar);
-    exit_protocol64(a);
     return;
   }
 
@@ -1493,7 +1485,6 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   __syncthreads();
   store_slab(X, RS, nb, nloc, a.A + r0, a.lda, nullptr);
-  exit_protocol64(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1936,7 +1927,7 @@ size_t panel_smem_bytes(int nb, int rs) {
 }  // namespace
 
 // persistent scratch size (doubles) the runtime allocates and zeroes once
-extern "C" size_t ld_panel_scratch_doubles(void) { return SCRATCH_ALL; }
+extern "C" size_t ld_panel_scratch_doubles(void) { return 3 * (size_t)SCRATCH_ALL; }
 
 // mode: 0 column-by-column Householder, 1 Cholesky-QR2 + reconstruction
 // (with the column-by-column fallback inside)
@@ -2137,7 +2128,12 @@ extern "C" int ld_panel_qr64(int ml, int nb, double *A, int lda, double *T, int 
   pa.ml = ml; pa.nb = nb; pa.A = A; pa.lda = lda; pa.T = T; pa.ldt = ldt;
   pa.Q = nullptr; pa.ldq = 0; pa.G = G; pa.rp = rp; pa.rs = rs; pa.sr = rp;
   pa.decline = 0; pa.batch = 0;
-  pa.scr = ld_panel_scratch();
+  // alternate scratch sets (see clear_set64): this launch's and the next's
+  static int par64 = 0;
+  double *const scr0 = ld_panel_scratch();
+  pa.scr = scr0 + (size_t)SCRATCH_ALL * (1 + par64);
+  pa.scr_next = scr0 + (size_t)SCRATCH_ALL * (2 - par64);
+  par64 ^= 1;
   int *flags = ld_device_flags();
   pa.cnt = (unsigned *)(flags + 8);
   pa.bar = (unsigned *)(pa.scr + OFF_BAR);
