@@ -698,6 +698,33 @@ __device__ bool near_identity32(double *M, double *MI, int *flagp) {
   return ok;
 }
 
+// A square panel (ml == nb): its last column has no rows below the
+// diagonal, and dgeqr3's dlarfg returns tau = 0 for it (H = I; R's entry
+// keeps the sign it has).  The LU (row-major M) ran over the first nb - 1
+// columns: column j = nb - 1 was eliminated by them, not itself; its
+// diagonal holds the Schur complement t, +-1 up to rounding since H_j is
+// orthogonal, and its U entries above carry no sign yet.  S_j = sign(t)
+// makes the pivot S_j t - 1 = -tau_j zero: the pivot is set to exactly 0
+// (D_j to 0) and the column's U entries take the sign.  False (nothing
+// changed) when |t| is not within 1e-8 of 1.  Every thread returns the same.
+__device__ bool square_last32(double *M, double *S, double *D, int nb) {
+  const int tid = threadIdx.x, j = nb - 1;
+  const double t = M[j * LDM + j];
+  const bool good = fabs(fabs(t) - 1.0) <= 1e-8;
+  const double s = (t > 0.0) ? 1.0 : -1.0;
+  __syncthreads();  // t read everywhere before it changes
+  if (good) {
+    if (tid < j) M[tid * LDM + j] *= s;
+    if (tid == j) {
+      M[j * LDM + j] = 0.0;
+      S[j] = s;
+      D[j] = 0.0;
+    }
+  }
+  __syncthreads();
+  return good;
+}
+
 // the fallback path out of line, so that its registers do not weigh on the
 // Cholesky-QR path's allocation
 __device__ __noinline__ void hh_fallback(const PanelArgs &a, double *slab, double *tail, int r0,
@@ -796,11 +823,17 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       M2[r * LDM + c] = (r < nb && c < nb) ? M3[r + c * LDM] : (r == c ? 1.0 : 0.0);
     }
     __syncthreads();
+    const bool square = a.ml == nb;
     if (wave == 0) {
-      const bool ok = wave_lu_sign32(M2, Sv, D2, nb);
+      const bool ok = wave_lu_sign32(M2, Sv, D2, square ? nb - 1 : nb);
       if (tid == 0) flag[0] = ok ? 1 : 0;
     }
     __syncthreads();
+    if (square && flag[0]) {
+      const bool good = square_last32(M2, Sv, D2, nb);
+      if (tid == 0 && !good) flag[0] = 0;
+      __syncthreads();
+    }
   }
 
   if (!flag[0]) {
@@ -1092,6 +1125,28 @@ __device__ void gram64_read(const double *G, int nb, double *M) {
     M[((r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)] = v;
   }
   __syncthreads();
+}
+
+// square_last32 for the LU in 4-block storage (the 64-column panel)
+__device__ bool square_last64(double *M, double *S, double *D, int nb) {
+  const int tid = threadIdx.x, j = nb - 1;
+  auto at = [M](int r, int c) -> double & {
+    return M[(2 * (r >> 5) + (c >> 5)) * B32 + (r & 31) * LDM + (c & 31)];
+  };
+  const double t = at(j, j);
+  const bool good = fabs(fabs(t) - 1.0) <= 1e-8;
+  const double s = (t > 0.0) ? 1.0 : -1.0;
+  __syncthreads();  // t read everywhere before it changes
+  if (good) {
+    if (tid < j) at(tid, j) *= s;
+    if (tid == j) {
+      at(j, j) = 0.0;
+      S[j] = s;
+      D[j] = 0.0;
+    }
+  }
+  __syncthreads();
+  return good;
 }
 
 // sign-chosen LU of Q_top (4-block storage in M, identity beyond nb) in
@@ -1402,7 +1457,9 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
     }
     // the sign-chosen LU of Q_top (M0 free in every workgroup: scratch)
-    ok = wg_lu64(M2, Sv, D2, M0, one, nb, flag);
+    const bool square = a.ml == nb;
+    ok = wg_lu64(M2, Sv, D2, M0, one, square ? nb - 1 : nb, flag);
+    if (ok && square) ok = square_last64(M2, Sv, D2, nb);
   }
 
   if (!ok) {

@@ -167,3 +167,30 @@ def test_second_pass_near_identity_or_cholesky(lila, ora, width, kappa):
     Qx = np.eye(m, n) - V @ (np.triu(Tg) @ V[:n].T)
     assert np.linalg.norm(Qx @ np.triu(Ag[:n]) - A) / np.linalg.norm(A) < 100 * n * U64
     assert np.linalg.norm(Qx.T @ Qx - np.eye(n)) < 100 * n * U64
+
+
+@pytest.mark.parametrize("width,n", [(64, 64), (64, 48), (64, 33), (32, 32), (32, 20), (32, 1)])
+def test_square_panel_last_reflector(lila, ora, width, n):
+    """a square panel: its last column has no rows below the diagonal, so
+    dgeqr3's last reflector is H = I (tau = 0: T's last diagonal entry is
+    exactly 0) and R's last diagonal entry keeps the sign of the value the
+    earlier reflectors leave there.  The reconstruction takes that column's
+    sign from the LU's last Schur complement (+-1) instead of the column
+    loop; V, R, T as the oracle's, within kappa n u.  (The 64-wide launch
+    takes 64 rows and more, so a square panel there is 64 x 64; narrower
+    square matrices end in a square 32-wide panel of the recursion.)"""
+    lila.call("lila_set_panel_width", width)
+    A = ora.fill_normal(5 + n + width, n, n)
+    kappa = np.linalg.cond(A)
+    say("square panel, width", width, "n", n, "kappa %.3g" % kappa)
+    (Ao, To), (Ag, Tg), n64 = run_dgeqr3(lila, ora, A)
+    assert n64 == (1 if width == 64 and n == 64 else 0)
+    assert To[n - 1, n - 1] == 0.0 and Tg[n - 1, n - 1] == 0.0
+    tol = 50 * n * U64 * kappa
+    assert rel(Ag, Ao) < tol
+    assert rel(np.triu(Tg), np.triu(To)) < tol
+    assert np.sign(Ag[n - 1, n - 1]) == np.sign(Ao[n - 1, n - 1])
+    V = np.tril(Ag, -1) + np.eye(n)
+    Qx = np.eye(n) - V @ (np.triu(Tg) @ V.T)
+    assert np.linalg.norm(Qx @ np.triu(Ag) - A) / np.linalg.norm(A) < 100 * n * U64
+    assert np.linalg.norm(Qx.T @ Qx - np.eye(n)) < 100 * n * U64
