@@ -426,6 +426,12 @@ int lila_set_panel_width(int nb);
  * after the VRT recursion by whole T blocks from the last to the first (the
  * same Q to rounding).  Returns the previous setting. */
 int lila_set_q_after(int on);
+/* dgeqr3 nodes whose halves are at most 64 columns wide: 1 (default) the
+ * update of the right half and the T join each in one persistent launch
+ * (partial products per workgroup, a fixed-order sum, the 64 x 64 work); 0
+ * as GEMM launches.  The same V, R, T to rounding.  Returns the previous
+ * setting.  (An extension: the reference has no counterpart.) */
+int lila_set_node_fused(int on);
 
 /* Cholesky-QR of the hr paths (lila_dgeqrf_w03_mt_hr, lila_dgeqr2_w03_hr
  * and the hr leaves; returns the previous setting, -1 outside 0..3): 0

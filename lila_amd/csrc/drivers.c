@@ -249,6 +249,8 @@ static void panel(int ml, int nb, double *A, int lda, double *T, int ldt, double
  * up to 32 go to the fused panel kernel. */
 /* the panel width: widths up to it go to one panel launch (the autotuner's
  * choice, 8..32) */
+static int node_kernels(void) { return ld_tune()->node_fused; }
+
 static int panel_width(void) {
     const int w = ld_tune()->panel_nb;
     return (w >= 8 && w <= LD_PANEL_MAX) ? w : LD_PANEL_MAX;
@@ -264,9 +266,14 @@ int dv_dgeqr3(int m, int n, double *A, int lda, double *T, int ldt) {
         return 0;
     }
     int n1 = n / 2, n2 = n - n1;
+    const int node = n1 <= 64 && n2 <= 64 && node_kernels();
     dv_dgeqr3(m, n1, A, lda, T, ldt);
-    apply_block_qt(m, n2, n1, A, lda, T, ldt, A + (size_t)n1 * lda, lda);
+    if (!node || ld_node_update64(m, n1, n2, A, lda, T, ldt, A + (size_t)n1 * lda, lda) != 0)
+        apply_block_qt(m, n2, n1, A, lda, T, ldt, A + (size_t)n1 * lda, lda);
     dv_dgeqr3(m - n1, n2, A + n1 + (size_t)n1 * lda, lda, T + n1 + (size_t)n1 * ldt, ldt);
+    if (node && ld_node_join64(m - n1, n1, n2, A + n1, lda, A + n1 + (size_t)n1 * lda, lda, T, ldt,
+                               T + n1 + (size_t)n1 * ldt, ldt, T + (size_t)n1 * ldt, ldt) == 0)
+        return 0;
     size_t mark = ld_ws_mark();
     double *H = wsd((size_t)n1 * n2);
     gemm_then_tri(T_, N_, n1, n2, m - n1, A + n1, lda, LD_GEN, A + n1 + (size_t)n1 * lda, lda,

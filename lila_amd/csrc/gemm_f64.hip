@@ -644,10 +644,13 @@ static void gemm2_general(const ld_gemm_t *g) {
   {
     // split K when the tiles give fewer workgroups than the split-K target
     // (512 by default: two per CU, the kernel's occupancy), down to 256 k
-    // per split: the partials are summed and scaled by the split-K reduction
+    // per split: the partials are summed and scaled by the split-K reduction.
+    // From three quarters of the target on, the tiles alone nearly fill the
+    // chip and the reduction's pass over the partials (2 M N doubles and
+    // more) costs more than the split gains
     int splits = 1, kchunk = Ki;
     const long target = ld_tune()->gemm_min_wgs;
-    if (tiles < target && Ki >= 512) {
+    if (tiles < target && 4 * tiles < 3 * target && Ki >= 512) {
       splits = (int)((target + tiles - 1) / tiles);
       if (splits > Ki / 256) splits = Ki / 256;
       const long cap = (64L << 20) / ((long)Mi * Ni);

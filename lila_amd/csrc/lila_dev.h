@@ -74,6 +74,14 @@ void ld_panel_qr(int ml, int n, double *A, int lda, double *T, int ldt,
 /* the 64-wide Cholesky-QR2 panel (32 < n <= 64, no explicit Q, panel mode
  * 1): 0 launched, -1 not taken (nothing launched) */
 int ld_panel_qr64(int ml, int n, double *A, int lda, double *T, int ldt);
+/* the two block steps of a dgeqr3 node whose halves are at most 64 wide,
+ * one launch each (node_qt.hip); 0 launched, -1 not taken (nothing
+ * launched).  update: C (ml x n2) -= V (T1^T (V^T C)), V ml x n1 unit lower;
+ * join: T12 (n1 x n2) = -T1 (V1^T V2) T2, V1 ml2 x n1, V2 ml2 x n2 unit lower */
+int ld_node_update64(int ml, int n1, int n2, const double *V, int ldv, const double *T1, int ldt,
+                     double *C, int ldc);
+int ld_node_join64(int ml2, int n1, int n2, const double *V1, int ldv1, const double *V2, int ldv2,
+                   const double *T1, int ldt1, const double *T2, int ldt2, double *T12, int ldt12);
 /* tall panels (taller than the workgroups' LDS slabs hold): */
 int ld_panel_is_tall(int ml, int n);
 /* the streamed Cholesky-QR2 panel without its column-loop fallback: 0 done,
@@ -213,6 +221,8 @@ typedef struct {
     int gemm_v2;             /* 1: general products on the 128 x 128 direct-to-LDS kernel */
     int q_after;             /* VRTQ recursion: 1 builds Q after the VRT recursion by whole T
                                 blocks, 0 at every node as the reference's call tree does */
+    int node_fused;          /* dgeqr3 nodes with halves <= 64 wide: update and T join in
+                                one launch each (node_qt.hip) */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

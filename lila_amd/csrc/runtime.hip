@@ -10,6 +10,7 @@
 #include "lila_dev.h"
 
 extern "C" size_t ld_panel_scratch_doubles(void);
+extern "C" size_t ld_node_scratch_doubles(void);
 
 #define HIPCHECK(x)                                                           \
   do {                                                                          \
@@ -51,7 +52,8 @@ struct State {
   int info = 0;
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
-  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64, 0, 64, 0, 0, 1, 0};
+  double *nscratch = nullptr;  // node kernels' partial slots and barrier counters
+  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64, 0, 64, 0, 0, 1, 0, 1};
   int failed = 0;  // a workspace allocation of the current call failed
 };
 State g;
@@ -72,6 +74,9 @@ int ld_init(void) {
     const size_t pb = ld_panel_scratch_doubles() * sizeof(double);
     if (hipMalloc(&g.pscratch, pb) != hipSuccess) return -1;
     if (hipMemset(g.pscratch, 0, pb) != hipSuccess) return -1;
+    const size_t nb = ld_node_scratch_doubles() * sizeof(double);
+    if (hipMalloc(&g.nscratch, nb) != hipSuccess) return -1;
+    if (hipMemset(g.nscratch, 0, nb) != hipSuccess) return -1;
   }
   if (hipStreamCreateWithFlags(&g.aux_stream, hipStreamNonBlocking) != hipSuccess) return -1;
   for (int k = 0; k < NEV; k++)
@@ -108,6 +113,8 @@ int ld_init(void) {
   if (e) g.tune.gemm_v2 = atoi(e);
   e = getenv("LILA_Q_AFTER");
   if (e) g.tune.q_after = atoi(e) ? 1 : 0;
+  e = getenv("LILA_NODE_FUSED");
+  if (e) g.tune.node_fused = atoi(e) ? 1 : 0;
   e = getenv("LILA_GEMM_KMIN");
   if (e) g.tune.gemm_kmin = atoi(e);
   e = getenv("LILA_PANEL_MODE");
@@ -335,6 +342,7 @@ void ld_clear_info(void) { g.info = 0; }
 
 int *ld_device_flags(void) { return g.dflags; }
 double *ld_panel_scratch(void) { return g.pscratch; }
+double *ld_node_scratch(void) { return g.nscratch; }
 
 int ld_check_device_flags(void) {
   int h[4] = {0, 0, 0, 0};
@@ -479,6 +487,13 @@ int lila_set_q_after(int on) {
   if (ld_init() != 0) return -1;
   const int old = g.tune.q_after;
   g.tune.q_after = on ? 1 : 0;
+  return old;
+}
+
+int lila_set_node_fused(int on) {
+  if (ld_init() != 0) return -1;
+  const int old = g.tune.node_fused;
+  g.tune.node_fused = on ? 1 : 0;
   return old;
 }
 
