@@ -20,6 +20,7 @@
 #include <type_traits>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include "lila_dev.h"
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
@@ -544,6 +545,79 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
   }
 }
 
+// ---------------------------------------------------------------------------
+// Small products (M, N <= 512, K <= 512): the products of the recursion's
+// block steps with a T factor (T1^T W, T1 X, H T2 at 128..512 columns) are
+// a few to a few hundred megaflops each, and the tiled kernel's k loop makes
+// them a chain of global-load latencies (12-17 us, plus a split-K reduction
+// at 512).  Here a workgroup owns a 32 x 32 tile of C and stages op(A)'s 32
+// rows and op(B)'s 32 columns for up to 256 k at once (one batch of loads
+// per operand, the structure masks applied on the way), then each wave runs
+// the MFMA chain of its 16 x 16 quarter; K > 256 takes a second round.
+// LDS: [k][32] per operand (row stride SG_LD), read conflict-free by the
+// MFMA operand layout (16 consecutive doubles per k).
+// ---------------------------------------------------------------------------
+constexpr int SG_T = 32;          // output tile
+constexpr int SG_KC = 256;        // k per LDS round
+constexpr int SG_KMAX = 512;      // K of the products taken
+constexpr int SG_MAX = 512;       // M, N of the products taken
+constexpr int SG_LD = SG_T + 2;   // LDS row stride (doubles)
+constexpr int SG_LDS = 2 * SG_KC * SG_LD * 8;
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(NT) void gemm_small_kernel(int M, int N, int K, double alpha, double beta,
+                                                        const double *__restrict__ A, int lda, int sa,
+                                                        const double *__restrict__ B, int ldb, int sb,
+                                                        double *__restrict__ C, int ldc, int zero_rows) {
+  extern __shared__ double sg_lds[];
+  double *As = sg_lds, *Bs = sg_lds + SG_KC * SG_LD;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int wi = wave & 1, wj = wave >> 1;
+  const int i0 = blockIdx.x * SG_T, j0 = blockIdx.y * SG_T;
+  d4_t acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += SG_KC) {
+    const int kc = min(SG_KC, K - k0);
+    if (k0 > 0) __syncthreads();  // the previous round's reads are done
+    double va[SG_T * SG_KC / NT], vb[SG_T * SG_KC / NT];
+#pragma unroll
+    for (int u = 0; u < SG_T * SG_KC / NT; u++) {
+      const int e = t + NT * u;
+      // op(A)(i, k): along i when A is stored M x K, along k when K x M
+      const int ia = TA ? (e >> 8) : (e & 31), ka = TA ? (e & 255) : (e >> 5);
+      const int gi = i0 + ia, gk = k0 + ka;
+      va[u] = (gi < M && ka < kc) ? (TA ? mask_val(A, lda, gk, gi, sa) : mask_val(A, lda, gi, gk, sa)) : 0.0;
+      // op(B)(k, j): along k when B is stored K x N, along j when N x K
+      const int jb = TB ? (e & 31) : (e >> 8), kb = TB ? (e >> 5) : (e & 255);
+      const int gj = j0 + jb, gkb = k0 + kb;
+      vb[u] = (gj < N && kb < kc) ? (TB ? mask_val(B, ldb, gj, gkb, sb) : mask_val(B, ldb, gkb, gj, sb)) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SG_T * SG_KC / NT; u++) {
+      const int e = t + NT * u;
+      const int ia = TA ? (e >> 8) : (e & 31), ka = TA ? (e & 255) : (e >> 5);
+      const int jb = TB ? (e & 31) : (e >> 8), kb = TB ? (e >> 5) : (e & 255);
+      As[ka * SG_LD + ia] = va[u];
+      Bs[kb * SG_LD + jb] = vb[u];
+    }
+    __syncthreads();
+    const int steps = (kc + 3) >> 2;
+    for (int s = 0; s < steps; s++) {
+      const int k = 4 * s + lk;
+      acc = mfma16(As[k * SG_LD + 16 * wi + li], Bs[k * SG_LD + 16 * wj + li], acc);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int r = i0 + 16 * wi + lk + 4 * v, c = j0 + 16 * wj + li;
+    if (r < M && c < N) {
+      double *cp = &C[(size_t)r + (size_t)c * ldc];
+      double o = alpha * acc[v];
+      if (beta != 0.0 && r >= zero_rows) o += beta * (*cp);
+      *cp = o;
+    }
+  }
+}
+
 }  // namespace
 
 // tile shape and split-K count of a product
@@ -757,6 +831,38 @@ extern "C" void ld_gemm(const ld_gemm_t *g) {
   gemm_tiles(g);
 }
 
+// the small-product kernel for M, N <= 512, K <= 512 (FP64, no SYRK form);
+// false when the product is not one it takes (LILA_GEMM_SMALL=0 turns it off)
+static bool gemm_small(const ld_gemm_t *g, hipStream_t st) {
+  static int on = -1;
+  if (on < 0) {
+    const char *e = getenv("LILA_GEMM_SMALL");
+    on = e ? (atoi(e) != 0) : 1;
+    if (on) {
+      for (const void *f : {(const void *)gemm_small_kernel<false, false>, (const void *)gemm_small_kernel<false, true>,
+                            (const void *)gemm_small_kernel<true, false>, (const void *)gemm_small_kernel<true, true>})
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, SG_LDS) != hipSuccess) {
+          (void)hipGetLastError();
+          on = 0;
+        }
+    }
+  }
+  if (!on || g->m > SG_MAX || g->n > SG_MAX || g->k > SG_KMAX || g->c_upper || ld_tune()->gemm_lowp) return false;
+  const dim3 grid((g->m + SG_T - 1) / SG_T, (g->n + SG_T - 1) / SG_T);
+  const int ph = ld_prof_begin("gemm_small_kernel");
+#define SGO(TA_, TB_)                                                                                     \
+  hipLaunchKernelGGL((gemm_small_kernel<TA_, TB_>), grid, dim3(NT), SG_LDS, st, g->m, g->n, g->k, g->alpha, \
+                     g->beta, g->A, g->lda, g->sa, g->B, g->ldb, g->sb, g->C, g->ldc, g->zero_rows)
+  if (!g->ta && !g->tb) SGO(false, false);
+  else if (!g->ta && g->tb) SGO(false, true);
+  else if (g->ta && !g->tb) SGO(true, false);
+  else SGO(true, true);
+#undef SGO
+  const double M = g->m, N = g->n, K = g->k;
+  ld_prof_end(ph, 2.0 * M * N * K, 8.0 * (M * K + K * N + 2.0 * M * N));
+  return true;
+}
+
 static void gemm_tiles(const ld_gemm_t *g0) {
   if (g0->m <= 0 || g0->n <= 0) return;
   // the tiled kernel knows LD_UPPER_Z as LD_UPPER (it masks the lower part)
@@ -773,6 +879,7 @@ static void gemm_tiles(const ld_gemm_t *g0) {
     return;
   }
   const int M = g->m, N = g->n;
+  if (gemm_small(g, st)) return;
   int bm, bn, splits, kchunk;
   gemm_plan(g, bm, bn, splits, kchunk);
   double *P = nullptr;

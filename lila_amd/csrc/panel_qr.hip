@@ -806,7 +806,8 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int r = ti * 16 + (lane >> 4) + 4 * v, c = tj * 16 + (lane & 15);
-      if (r <= c && c < nb) a.T[(size_t)r + (size_t)c * a.ldt] = -tacc[v];
+      // a square panel's last reflector is H = I: its T column is exactly 0
+      if (r <= c && c < nb) a.T[(size_t)r + (size_t)c * a.ldt] = (a.ml == nb && c == nb - 1) ? 0.0 : -tacc[v];
     }
   }
   __syncthreads();
@@ -1463,7 +1464,8 @@ __global__ __launch_bounds__(PNT) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int r = ti * 16 + (lane >> 4) + 4 * v, c = tj * 16 + (lane & 15);
-        if (r <= c && c < nb) a.T[(size_t)r + (size_t)c * a.ldt] = -acc[v];
+        // a square panel's last reflector is H = I: its T column is exactly 0
+        if (r <= c && c < nb) a.T[(size_t)r + (size_t)c * a.ldt] = (a.ml == nb && c == nb - 1) ? 0.0 : -acc[v];
       }
     }
   }

@@ -43,8 +43,11 @@ def sync_ok(lila):
 @pytest.mark.parametrize("ta", ["N", "T"])
 @pytest.mark.parametrize("tb", ["N", "T"])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (128, 128, 16), (130, 70, 300),
+                                   (512, 512, 512), (500, 300, 257), (33, 480, 511), (512, 513, 64),
                                    (64, 96, 20000), (1000, 513, 64), (32, 32, 100000)])
 def test_dgemm(lila, ora, ta, tb, m, n, k):
+    # M, N <= 512 with K <= 512 take the one-shot small-product kernel (all of
+    # K staged per round of 256); the rest the tiled and direct-to-LDS kernels
     say("dgemm", ta, tb, m, n, k)
     rng = np.random.default_rng(m * 7 + n * 3 + k)
     A = np.asfortranarray(rng.standard_normal((k, m) if ta == "T" else (m, k)))
