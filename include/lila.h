@@ -432,6 +432,14 @@ int lila_set_q_after(int on);
  * as GEMM launches.  The same V, R, T to rounding.  Returns the previous
  * setting.  (An extension: the reference has no counterpart.) */
 int lila_set_node_fused(int on);
+/* Backend of the large plain products (no structure, op(A) = A, op(B) = B:
+ * the general interiors of the trailing updates C -= V W and of lila_dgemm):
+ * 0 (default) the library's own direct-to-LDS FP64 MFMA kernel, 1 rocBLAS
+ * dgemm on the library stream.  Structured, transposed and small
+ * products always run on the library's kernels.  Returns the previous
+ * setting, -1 out of range.  (An extension: the reference has no
+ * counterpart.) */
+int lila_set_gemm_backend(int b);
 
 /* Cholesky-QR of the hr paths (lila_dgeqrf_w03_mt_hr, lila_dgeqr2_w03_hr
  * and the hr leaves; returns the previous setting, -1 outside 0..3): 0

@@ -49,7 +49,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(_run, jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lrocblas"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

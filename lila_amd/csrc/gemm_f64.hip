@@ -22,6 +22,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include "lila_dev.h"
+#include <rocblas/rocblas.h>
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -694,7 +695,36 @@ static double flops_upper_cols(int Ni, int Ki) {
   return s;
 }
 
+// A plain NN product (no structure, no transposes, no zero rows): the
+// interior the dispatcher sends to the direct-to-LDS kernel goes to rocBLAS
+// instead when the backend setting asks for it (lila_set_gemm_backend(1);
+// rocBLAS's dgemm measured 69-70 TFLOP/s on the trailing-update shapes
+// where gemm2_f64_kernel reaches 63-66).  False when not taken.
+static bool gemm_rocblas(const ld_gemm_t *g) {
+  static int state = 0;  // 0 not tried, 1 handle ready, -1 unavailable
+  static rocblas_handle hd = nullptr;
+  if (!ld_tune()->gemm_rocblas || g->sa != LD_GEN || g->sb != LD_GEN || g->zero_rows || g->c_upper || g->ta ||
+      g->tb)
+    return false;
+  if (state == 0) {
+    state = (rocblas_create_handle(&hd) == rocblas_status_success) ? 1 : -1;
+    if (state < 0) fprintf(stderr, "lila_hip: no rocBLAS handle; plain products stay on gemm2_f64_kernel\n");
+  }
+  if (state < 0 || rocblas_set_stream(hd, (hipStream_t)ld_stream()) != rocblas_status_success) return false;
+  const int ph = ld_prof_begin("rocblas_dgemm");
+  const rocblas_status st = rocblas_dgemm(hd, rocblas_operation_none, rocblas_operation_none, g->m, g->n, g->k,
+                                          &g->alpha, g->A, g->lda, g->B, g->ldb, &g->beta, g->C, g->ldc);
+  ld_prof_end(ph, 2.0 * g->m * g->n * (double)g->k,
+              8.0 * ((double)g->m * g->k + (double)g->k * g->n + 2.0 * g->m * g->n));
+  if (st != rocblas_status_success) {
+    fprintf(stderr, "lila_hip: rocblas_dgemm failed (%d)\n", (int)st);
+    ld_set_info(-1);
+  }
+  return true;
+}
+
 static void gemm2_general(const ld_gemm_t *g) {
+  if (gemm_rocblas(g)) return;
   const int M = g->m, N = g->n, K = g->k;
   const int Mi = M / G2_BM * G2_BM, Ni = N / G2_BN * G2_BN, Ki = K / G2_BK * G2_BK;
   const long tiles = (long)(Mi / G2_BM) * (Ni / G2_BN);

@@ -223,6 +223,8 @@ typedef struct {
                                 blocks, 0 at every node as the reference's call tree does */
     int node_fused;          /* dgeqr3 nodes with halves <= 64 wide: update and T join in
                                 one launch each (node_qt.hip) */
+    int gemm_rocblas;        /* 1: plain NN interiors of the large products through rocBLAS,
+                                0 (default): on gemm2_f64_kernel */
 } ld_tune_t;
 ld_tune_t *ld_tune(void);
 

@@ -53,7 +53,7 @@ struct State {
   int *dflags = nullptr;  // device flags: [0] barrier timeout, [1] potrf info, [8..9] barrier
   double *pscratch = nullptr;  // panel reduction vectors (kept zero between launches)
   double *nscratch = nullptr;  // node kernels' partial slots and barrier counters
-  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64, 0, 64, 0, 0, 1, 0, 1};
+  ld_tune_t tune = {384, 128, 512, 64, 64, 128, 256, 1, 0, 128, 128, 0, 64, 0, 64, 0, 0, 1, 0, 1, 0};
   int failed = 0;  // a workspace allocation of the current call failed
 };
 State g;
@@ -113,6 +113,8 @@ int ld_init(void) {
   if (e) g.tune.gemm_v2 = atoi(e);
   e = getenv("LILA_Q_AFTER");
   if (e) g.tune.q_after = atoi(e) ? 1 : 0;
+  e = getenv("LILA_ROCBLAS");
+  if (e) g.tune.gemm_rocblas = atoi(e) ? 1 : 0;
   e = getenv("LILA_NODE_FUSED");
   if (e) g.tune.node_fused = atoi(e) ? 1 : 0;
   e = getenv("LILA_GEMM_KMIN");
@@ -487,6 +489,14 @@ int lila_set_q_after(int on) {
   if (ld_init() != 0) return -1;
   const int old = g.tune.q_after;
   g.tune.q_after = on ? 1 : 0;
+  return old;
+}
+
+int lila_set_gemm_backend(int b) {
+  if (ld_init() != 0) return -1;
+  if (b != 0 && b != 1) return -1;
+  const int old = g.tune.gemm_rocblas;
+  g.tune.gemm_rocblas = b;
   return old;
 }
 

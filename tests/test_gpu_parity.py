@@ -67,14 +67,24 @@ def test_dgemm(lila, ora, ta, tb, m, n, k):
     assert np.all(np.abs(got - ref) <= bound + 1e-300)
 
 
+@pytest.mark.parametrize("backend", [0, 1])
 @pytest.mark.parametrize("ta", ["N", "T"])
 @pytest.mark.parametrize("m,n,k", [(2048, 2048, 512), (2100, 2200, 530), (1024, 8192, 3000),
                                    (4096, 1152, 16)])
-def test_dgemm_direct_lds_tiles(lila, ora, ta, m, n, k):
+def test_dgemm_direct_lds_tiles(lila, ora, ta, m, n, k, backend):
     """shapes large enough for the 128 x 128 direct-to-LDS kernel: its
-    interior, the ragged strips and the K tail (run by the tiled kernel) must
+    interior (backend 0; with backend 1 the plain NN interior runs on
+    rocBLAS), the ragged strips and the K tail (run by the tiled kernel) must
     add up to the product"""
-    say("dgemm 128x128 direct-to-LDS", ta, m, n, k)
+    say("dgemm 128x128 interior", ta, m, n, k, "backend", backend)
+    old = lila.call("lila_set_gemm_backend", backend)
+    try:
+        run_direct_lds(lila, ta, m, n, k)
+    finally:
+        lila.call("lila_set_gemm_backend", old)
+
+
+def run_direct_lds(lila, ta, m, n, k):
     rng = np.random.default_rng(m + 3 * n + 7 * k)
     A = np.asfortranarray(rng.standard_normal((k, m) if ta == "T" else (m, k)))
     B = np.asfortranarray(rng.standard_normal((k, n)))
@@ -196,6 +206,20 @@ def test_recursive_q_after(lila, ora, case, leaf, panel):
     finally:
         lila.call("lila_set_q_after", old)
     check_vrtq(ora, A, o, g, m, n, ii, mt, lda or m, ldq or m)
+
+
+@pytest.mark.parametrize("backend", [0, 1])
+def test_recursive_gemm_backends(lila, ora, backend):
+    """the plain NN interiors of the trailing updates on the library's
+    direct-to-LDS kernel (0) or on rocBLAS (1): the same V, R, T and Q"""
+    m, n, ii, mt, nx = 3000, 2400, 0, 512, 256
+    say("recursive, gemm backend", backend)
+    old = lila.call("lila_set_gemm_backend", backend)
+    try:
+        A, o, g = run_driver(lila, ora, "recursive", m, n, ii, mt, [0, 1, 1, nx])
+    finally:
+        lila.call("lila_set_gemm_backend", old)
+    check_vrtq(ora, A, o, g, m, n, ii, mt, m, m)
 
 
 @pytest.mark.parametrize("nb_lvl", [[32], [128], [64, 16]])
