@@ -428,7 +428,7 @@ __device__ __forceinline__ void g2_load(const double *gp, double *lp) {
   __builtin_amdgcn_global_load_lds((const void *)gp, (lds_void *)lp, 16, 0, 0);
 }
 
-template <bool TA>
+template <bool TA, bool KUP = false>
 __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, double alpha, double beta,
                                                          const double *__restrict__ A, int lda,
                                                          const double *__restrict__ B, int ldb,
@@ -453,6 +453,11 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
   const int wm = wave & 1, wn = wave >> 1;
   const int m0 = tx * G2_BM, n0 = ty * G2_BN;
   const int kbeg = blockIdx.z * kchunk;
+  // first column (within the tile) of the wave's 16-column tile b: 64
+  // contiguous columns per wave, or with KUP the two waves' tiles
+  // interleaved (0, 32, 64, 96 and 16, 48, 80, 112), so that both meet the
+  // triangle's skipped tiles alike and neither waits for the other
+  auto g2_col = [&](int wnn, int b) { return KUP ? wnn * 16 + b * 32 : wnn * 64 + b * 16; };
   // an upper triangular B with zeros below its diagonal: rows of B past the
   // tile's last column add nothing
   const int kend = min(kupper ? min(K, n0 + G2_BN) : K, kbeg + kchunk);
@@ -499,7 +504,7 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
       }
 #pragma unroll
       for (int b = 0; b < 4; b++) {
-        const int n = wn * 64 + b * 16 + r;
+        const int n = g2_col(wn, b) + r;
         fb[b] = bs[n * 16 + (((k >> 1) ^ ((n >> 1) & 7)) << 1) + (k & 1)];
       }
     };
@@ -509,9 +514,15 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
       const int cur = sstep & 1;
       if (sstep + 1 < G2_BK / 4) frags(sstep + 1, af[cur ^ 1], bf[cur ^ 1]);
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        // KUP: B upper triangular with zeros below the diagonal: rows of B
+        // past a 16-column tile's last column add nothing to it, so the wave
+        // skips that tile's MFMAs from there (a wave-uniform test; the freed
+        // MFMA cycles go to the CU's other workgroup)
+        if (KUP && kbeg + t * G2_BK + 4 * sstep > n0 + g2_col(wn, b) + 15) continue;
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = mfma16(bf[cur][b], af[cur][a], acc[a][b]);
+        for (int a = 0; a < 4; a++) acc[a][b] = mfma16(bf[cur][b], af[cur][a], acc[a][b]);
+      }
     }
   }
 
@@ -527,14 +538,14 @@ __global__ __launch_bounds__(NT, 2) void gemm2_f64_kernel(int M, int N, int K, d
     for (int b = 0; b < 4; b++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int gj = n0 + wn * 64 + b * 16 + kk + 4 * v;
+        const int gj = n0 + g2_col(wn, b) + kk + 4 * v;
         cv[b][v] = read_c ? C[(size_t)gi + (size_t)gj * ldc] : 0.0;
       }
 #pragma unroll
     for (int b = 0; b < 4; b++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int gj = n0 + wn * 64 + b * 16 + kk + 4 * v;
+        const int gj = n0 + g2_col(wn, b) + kk + 4 * v;
         if (P) {
           P[(size_t)blockIdx.z * M * N + (size_t)gi + (size_t)gj * M] = acc[a][b][v];
         } else {
@@ -735,7 +746,8 @@ static void gemm2_general(const ld_gemm_t *g) {
   hipStream_t st = (hipStream_t)ld_stream();
   static int attr = 0;
   if (!attr) {
-    for (const void *f : {(const void *)gemm2_f64_kernel<false>, (const void *)gemm2_f64_kernel<true>}) {
+    for (const void *f : {(const void *)gemm2_f64_kernel<false>, (const void *)gemm2_f64_kernel<true>,
+                          (const void *)gemm2_f64_kernel<false, true>}) {
       if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G2_LDS) != hipSuccess) {
         (void)hipGetLastError();
         ld_tune()->gemm_v2 = 0;
@@ -779,6 +791,9 @@ static void gemm2_general(const ld_gemm_t *g) {
     if (g->ta)
       hipLaunchKernelGGL(gemm2_f64_kernel<true>, grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha, g->beta,
                          g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P, kup);
+    else if (kup)
+      hipLaunchKernelGGL((gemm2_f64_kernel<false, true>), grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha,
+                         g->beta, g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P, kup);
     else
       hipLaunchKernelGGL(gemm2_f64_kernel<false>, grid, dim3(NT), G2_LDS, st, Mi, Ni, Ki, g->alpha, g->beta,
                          g->A, g->lda, g->B, g->ldb, g->C, g->ldc, kchunk, P, kup);
