@@ -344,10 +344,10 @@ __global__ void lu_sign_kernel(int n, int nrow, double *T, int ldt, double *Sg, 
 
 // rule 0 on an mr x nc panel (nc <= 32) by 256 threads: the top 32 x 32
 // (identity past the panel) by the one-wave sign-chosen LU -- the same
-// pivots, signs and factors as the column loop above -- and the rows below
-// it, whose L rows solve x U = a S (U with its signs, the pivots on its
-// diagonal), one row per thread
-__global__ __launch_bounds__(WG) void lu_sign32_leaf_kernel(int nc, int mr, double *T, int ldt, double *Sg) {
+// pivots, signs and factors as the column loop above; its reciprocal pivots
+// go to Dg for lu_rows_kernel, which solves the rows below
+__global__ __launch_bounds__(WG) void lu_sign32_leaf_kernel(int nc, int mr, double *T, int ldt, double *Sg,
+                                                            double *Dg) {
   __shared__ double M[B32], Sv[32], Dv[32];
   const int tid = threadIdx.x, wave = tid >> 6;
   for (int t = tid; t < 32 * 32; t += WG) {
@@ -362,22 +362,50 @@ __global__ __launch_bounds__(WG) void lu_sign32_leaf_kernel(int nc, int mr, doub
     if (r < mr && c < nc) T[(size_t)r + (size_t)c * ldt] = M[r * LDM + c];
   }
   if (tid < nc) Sg[tid] = Sv[tid];
-  // rows below the top 32: one row per thread, x U = a S by forward
-  // substitution against U in LDS (U(c, c) the pivot, 1 / pivot in Dv)
-  for (int r = 32 + tid; r < mr; r += WG) {
-    double x[32];
+  if (tid < 32) Dg[tid] = Dv[tid];
+}
+
+// the rows below the top 32 of lu_sign32_leaf_kernel's panel, LU_RB rows a
+// workgroup (a kernel of their own, the rows staged through LDS: computed
+// inline beside the one-wave LU from global addresses, a row's values and
+// their 32 addresses went to scratch memory, 60-140 us a leaf).  One row per
+// thread, x U = a S by forward substitution against U (the leaf's output rows
+// 0..31: strict upper part, signs applied) with the leaf's signs and
+// reciprocal pivots, right-looking: x_c2 -= x_c U(c, c2) for c < c2 in the
+// order of c, the same operations as the left-looking sum
+constexpr int LU_RB = 128, LU_LDX = LU_RB + 1;
+__global__ __launch_bounds__(LU_RB) void lu_rows_kernel(int nc, int mr, double *T, int ldt, const double *Sg,
+                                                        const double *Dg) {
+  __shared__ double U[32 * LDM], Sv[32], Dv[32], X[32 * LU_LDX];
+  const int tid = threadIdx.x, r0 = 32 + blockIdx.x * LU_RB, nr = min(LU_RB, mr - r0);
+  for (int t = tid; t < 32 * 32; t += LU_RB) {
+    const int r = t & 31, c = t >> 5;
+    U[r * LDM + c] = (r < c && c < nc) ? T[(size_t)r + (size_t)c * ldt] : 0.0;
+  }
+  if (tid < 32) {
+    Sv[tid] = (tid < nc) ? Sg[tid] : 1.0;
+    Dv[tid] = (tid < nc) ? Dg[tid] : 0.0;
+  }
+  for (int t = tid; t < 32 * LU_RB; t += LU_RB) {
+    const int rl = t % LU_RB, c = t / LU_RB;
+    X[c * LU_LDX + rl] = (rl < nr && c < nc) ? T[(size_t)(r0 + rl) + (size_t)c * ldt] : 0.0;
+  }
+  __syncthreads();
+  double x[32];
 #pragma unroll
-    for (int c = 0; c < 32; c++) x[c] = (c < nc) ? T[(size_t)r + (size_t)c * ldt] * Sv[c] : 0.0;
+  for (int c = 0; c < 32; c++) x[c] = X[c * LU_LDX + tid] * Sv[c];
 #pragma unroll
-    for (int c = 0; c < 32; c++) {
-      double acc = x[c];
+  for (int c = 0; c < 32; c++) {
+    x[c] *= Dv[c];
 #pragma unroll
-      for (int k = 0; k < c; k++) acc -= x[k] * M[k * LDM + c];
-      x[c] = acc * Dv[c];
-    }
+    for (int c2 = c + 1; c2 < 32; c2++) x[c2] -= x[c] * U[c * LDM + c2];
+  }
 #pragma unroll
-    for (int c = 0; c < 32; c++)
-      if (c < nc) T[(size_t)r + (size_t)c * ldt] = x[c];
+  for (int c = 0; c < 32; c++) X[c * LU_LDX + tid] = x[c];
+  __syncthreads();
+  for (int t = tid; t < 32 * LU_RB; t += LU_RB) {
+    const int rl = t % LU_RB, c = t / LU_RB;
+    if (rl < nr && c < nc) T[(size_t)(r0 + rl) + (size_t)c * ldt] = X[c * LU_LDX + rl];
   }
 }
 
@@ -586,10 +614,20 @@ void ld_n2t(int n, const double *tau, double *T, int ldt) {
 static void lu_sign_panel(int mr, int nc, double *A, int lda, double *Sg, int rule) {
   if (nc <= 0) return;
   if (nc <= 32) {
-    if (rule == 0)
-      hipLaunchKernelGGL(lu_sign32_leaf_kernel, dim3(1), dim3(WG), 0, S(), nc, mr, A, lda, Sg);
-    else
+    if (rule == 0) {
+      // the leaf's reciprocal pivots pass to the rows kernel through Dg
+      size_t mark = ld_ws_mark();
+      double *Dg = (double *)ld_ws_alloc(32 * sizeof(double));
+      if (!ld_failed()) {
+        hipLaunchKernelGGL(lu_sign32_leaf_kernel, dim3(1), dim3(WG), 0, S(), nc, mr, A, lda, Sg, Dg);
+        if (mr > 32)
+          hipLaunchKernelGGL(lu_rows_kernel, dim3((mr - 32 + LU_RB - 1) / LU_RB), dim3(LU_RB), 0, S(), nc, mr, A,
+                             lda, Sg, Dg);
+      }
+      ld_ws_release(mark);
+    } else {
       hipLaunchKernelGGL(lu_sign_kernel, dim3(1), dim3(1024), 0, S(), nc, mr, A, lda, Sg, rule);
+    }
     return;
   }
   const int n1 = (nc / 2 + 31) / 32 * 32, n2 = nc - n1;
