@@ -69,17 +69,26 @@ __global__ __launch_bounds__(WG) void trtri64_kernel(int nn, int upper, int unit
   __shared__ double M[3 * B32], X[3 * B32], Tm[B32], Dg[64];
   const int b = blockIdx.x, tid = threadIdx.x;
   const int o = b * TB, bs = min(TB, nn - o);
-  for (int t = tid; t < 64 * 64; t += WG) {
-    const int r = t & 63, c = t >> 6;  // M(r, c), upper
-    if (r >= 32 && c < 32) continue;
-    double v = 0.0;
+  // a thread's 16 values (row tid & 63, every fourth column) loaded into
+  // registers first, so that the loads are in flight together (a load-store
+  // loop waits out one memory latency per element)
+  double v[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = tid & 63, c = (tid >> 6) + 4 * i;  // M(r, c), upper
+    double x = 0.0;
     if (r < bs && c < bs) {
-      if (r < c) v = upper ? A[(size_t)(o + r) + (size_t)(o + c) * lda] : A[(size_t)(o + c) + (size_t)(o + r) * lda];
-      else if (r == c) v = unit ? 1.0 : A[(size_t)(o + r) + (size_t)(o + r) * lda];
+      if (r < c) x = upper ? A[(size_t)(o + r) + (size_t)(o + c) * lda] : A[(size_t)(o + c) + (size_t)(o + r) * lda];
+      else if (r == c) x = unit ? 1.0 : A[(size_t)(o + r) + (size_t)(o + r) * lda];
     } else if (r == c) {
-      v = 1.0;
+      x = 1.0;
     }
-    uel(M, r, c) = v;
+    v[i] = x;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = tid & 63, c = (tid >> 6) + 4 * i;
+    if (!(r >= 32 && c < 32)) uel(M, r, c) = v[i];
   }
   __syncthreads();
   if (tid < 64) Dg[tid] = 1.0 / uel(M, tid, tid);
@@ -185,10 +194,19 @@ __global__ __launch_bounds__(WG) void potf2_64_kernel(int bs, double *A, int lda
   __shared__ double M[3 * B32], Sc[B32], Dg[64];
   __shared__ int okf;
   const int tid = threadIdx.x, wave = tid >> 6;
-  for (int t = tid; t < 64 * 64; t += WG) {
-    const int r = t & 63, c = t >> 6;
-    if (r >= 32 && c < 32) continue;
-    uel(M, r, c) = (r < bs && c < bs) ? (r <= c ? A[(size_t)r + (size_t)c * lda] : 0.0) : (r == c ? 1.0 : 0.0);
+  {
+    // all of a thread's loads in flight together (as in trtri64_kernel)
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int r = tid & 63, c = (tid >> 6) + 4 * i;
+      v[i] = (r < bs && c < bs) ? (r <= c ? A[(size_t)r + (size_t)c * lda] : 0.0) : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int r = tid & 63, c = (tid >> 6) + 4 * i;
+      if (!(r >= 32 && c < 32)) uel(M, r, c) = v[i];
+    }
   }
   __syncthreads();
   if (wave == 0) {
@@ -350,9 +368,18 @@ __global__ __launch_bounds__(WG) void lu_sign32_leaf_kernel(int nc, int mr, doub
                                                             double *Dg) {
   __shared__ double M[B32], Sv[32], Dv[32];
   const int tid = threadIdx.x, wave = tid >> 6;
-  for (int t = tid; t < 32 * 32; t += WG) {
-    const int r = t & 31, c = t >> 5;
-    M[r * LDM + c] = (r < mr && c < nc) ? T[(size_t)r + (size_t)c * ldt] : (r == c ? 1.0 : 0.0);
+  {
+    double v[4];  // all four loads of a thread in flight together
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int t = tid + i * WG, r = t & 31, c = t >> 5;
+      v[i] = (r < mr && c < nc) ? T[(size_t)r + (size_t)c * ldt] : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int t = tid + i * WG, r = t & 31, c = t >> 5;
+      M[r * LDM + c] = v[i];
+    }
   }
   __syncthreads();
   if (wave == 0) (void)small32::wave_lu_sign32(M, Sv, Dv, nc);
@@ -378,28 +405,38 @@ __global__ __launch_bounds__(LU_RB) void lu_rows_kernel(int nc, int mr, double *
                                                         const double *Dg) {
   __shared__ double U[32 * LDM], Sv[32], Dv[32], X[32 * LU_LDX];
   const int tid = threadIdx.x, r0 = 32 + blockIdx.x * LU_RB, nr = min(LU_RB, mr - r0);
-  for (int t = tid; t < 32 * 32; t += LU_RB) {
-    const int r = t & 31, c = t >> 5;
-    U[r * LDM + c] = (r < c && c < nc) ? T[(size_t)r + (size_t)c * ldt] : 0.0;
+  // every load of a thread issued before the first is used: U's strict upper
+  // triangle (8 values a thread) and the thread's row (32 values; a wave
+  // reads 64 consecutive rows of a column at a time)
+  double u[8], x[32];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int t = tid + i * LU_RB, r = t & 31, c = t >> 5;
+    u[i] = (r < c && c < nc) ? T[(size_t)r + (size_t)c * ldt] : 0.0;
+  }
+  const bool row = tid < nr;
+  const double *xr = T + (size_t)(r0 + (row ? tid : 0));
+#pragma unroll
+  for (int c = 0; c < 32; c++) x[c] = (row && c < nc) ? xr[(size_t)c * ldt] : 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int t = tid + i * LU_RB, r = t & 31, c = t >> 5;
+    U[r * LDM + c] = u[i];
   }
   if (tid < 32) {
     Sv[tid] = (tid < nc) ? Sg[tid] : 1.0;
     Dv[tid] = (tid < nc) ? Dg[tid] : 0.0;
   }
-  for (int t = tid; t < 32 * LU_RB; t += LU_RB) {
-    const int rl = t % LU_RB, c = t / LU_RB;
-    X[c * LU_LDX + rl] = (rl < nr && c < nc) ? T[(size_t)(r0 + rl) + (size_t)c * ldt] : 0.0;
-  }
   __syncthreads();
-  double x[32];
 #pragma unroll
-  for (int c = 0; c < 32; c++) x[c] = X[c * LU_LDX + tid] * Sv[c];
+  for (int c = 0; c < 32; c++) x[c] *= Sv[c];
 #pragma unroll
   for (int c = 0; c < 32; c++) {
     x[c] *= Dv[c];
 #pragma unroll
     for (int c2 = c + 1; c2 < 32; c2++) x[c2] -= x[c] * U[c * LDM + c2];
   }
+  // through LDS to the stores, so that no address of the loads stays live
 #pragma unroll
   for (int c = 0; c < 32; c++) X[c * LU_LDX + tid] = x[c];
   __syncthreads();
