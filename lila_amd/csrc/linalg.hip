@@ -129,10 +129,108 @@ inline const double *opA(const TrsmCtx &t, int r0, int c0) {
   return t.tA ? t.A + c0 + (size_t)r0 * t.lda : t.A + r0 + (size_t)c0 * t.lda;
 }
 
+// a leaf of the blocked TRSM whose B is narrow, in one launch and in place:
+// B_o <- op(D) B_o (LEFT: a workgroup per 32 columns of B, all len rows) or
+// B_o <- B_o op(D) (right: a workgroup per 64 rows, all len columns), D the
+// leaf's inverted diagonal block (TB x TB, zero outside its triangle).  A
+// workgroup reads everything it writes before its first store, and no two
+// workgroups share an element, so the product needs no copy of B (the GEMM
+// form writes a workspace and copies it back: two launches).  Sums run in
+// the order of k, one FMA per term.
+constexpr int TL_LD = TB + 1;   // LDS row stride
+constexpr int TL_MAX = 2048;    // the narrow side's largest extent taken here
+template <bool LEFT>
+__global__ __launch_bounds__(WG) void trsm_leaf_kernel(int len, int other, int tA, const double *__restrict__ D,
+                                                       double *B, int ldb) {
+  __shared__ double Ds[TB * TL_LD], Bs[TB * TL_LD];
+  const int tid = threadIdx.x;
+  // op(D) into LDS: LEFT as Ds[r][k] = op(D)(r, k), right as Ds[k][c] =
+  // op(D)(k, c); every load of a thread in flight together
+  {
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int q = tid + i * WG;
+      v[i] = D[q];  // D(q & 63, q >> 6), column-major, ld TB
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int q = tid + i * WG, di = q & 63, dj = q >> 6;
+      // LEFT: op(D)(r, k) = tA ? D(k, r) : D(r, k); right: op(D)(k, c) = tA ? D(c, k) : D(k, c)
+      const int row = tA ? dj : di, col = tA ? di : dj;
+      Ds[row * TL_LD + col] = v[i];
+    }
+  }
+  if (LEFT) {
+    const int c0 = blockIdx.x * 32;
+    double v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int q = tid + i * WG, k = q & 63, c = q >> 6;
+      v[i] = (k < len && c0 + c < other) ? B[(size_t)k + (size_t)(c0 + c) * ldb] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int q = tid + i * WG, k = q & 63, c = q >> 6;
+      Bs[k * TL_LD + c] = v[i];
+    }
+    __syncthreads();
+    const int c = tid & 31, rg = tid >> 5;
+    double acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = 0.0;
+    for (int k = 0; k < len; k++) {
+      const double b = Bs[k * TL_LD + c];
+#pragma unroll
+      for (int i = 0; i < 8; i++) acc[i] = fma(Ds[(rg + 8 * i) * TL_LD + k], b, acc[i]);
+    }
+    if (c0 + c < other)
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+        if (rg + 8 * i < len) B[(size_t)(rg + 8 * i) + (size_t)(c0 + c) * ldb] = acc[i];
+  } else {
+    const int r0 = blockIdx.x * 64;
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int q = tid + i * WG, r = q & 63, k = q >> 6;
+      v[i] = (k < len && r0 + r < other) ? B[(size_t)(r0 + r) + (size_t)k * ldb] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int q = tid + i * WG, r = q & 63, k = q >> 6;
+      Bs[k * TL_LD + r] = v[i];
+    }
+    __syncthreads();
+    const int r = tid & 63, cg = tid >> 6;
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = 0.0;
+    for (int k = 0; k < len; k++) {
+      const double b = Bs[k * TL_LD + r];
+#pragma unroll
+      for (int i = 0; i < 16; i++) acc[i] = fma(b, Ds[k * TL_LD + cg + 4 * i], acc[i]);
+    }
+    if (r0 + r < other)
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        if (cg + 4 * i < len) B[(size_t)(r0 + r) + (size_t)(cg + 4 * i) * ldb] = acc[i];
+  }
+}
+
 // solve on the index range [o, o+len) of the triangle (multiple of TB at o)
 void trsm_rec(const TrsmCtx &t, int o, int len) {
   if (len <= TB) {
     const double *Db = t.D + (size_t)(o / TB) * TB * TB;
+    if ((t.side == 'L' ? t.n : t.m) <= TL_MAX) {
+      if (t.side == 'L')
+        hipLaunchKernelGGL(trsm_leaf_kernel<true>, dim3((t.n + 31) / 32), dim3(WG), 0, S(), len, t.n, t.tA, Db,
+                           t.B + o, t.ldb);
+      else
+        hipLaunchKernelGGL(trsm_leaf_kernel<false>, dim3((t.m + 63) / 64), dim3(WG), 0, S(), len, t.m, t.tA, Db,
+                           t.B + (size_t)o * t.ldb, t.ldb);
+      return;
+    }
     size_t mark = ld_ws_mark();
     if (t.side == 'L') {
       double *W = (double *)ld_ws_alloc(sizeof(double) * (size_t)len * t.n);
