@@ -491,9 +491,9 @@ __global__ __launch_bounds__(WG) void lu_sign32_leaf_kernel(int nc, int mr, doub
 }
 
 // the rows below the top 32 of lu_sign32_leaf_kernel's panel, LU_RB rows a
-// workgroup (a kernel of their own, the rows staged through LDS: computed
-// inline beside the one-wave LU from global addresses, a row's values and
-// their 32 addresses went to scratch memory, 60-140 us a leaf).  One row per
+// workgroup (a kernel of their own, the results stored through LDS: computed
+// inline beside the one-wave LU and stored from registers, a row's values
+// and their 32 addresses went to scratch memory, 60-140 us a leaf).  One row per
 // thread, x U = a S by forward substitution against U (the leaf's output rows
 // 0..31: strict upper part, signs applied) with the leaf's signs and
 // reciprocal pivots, right-looking: x_c2 -= x_c U(c, c2) for c < c2 in the
